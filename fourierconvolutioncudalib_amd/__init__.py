@@ -1,0 +1,196 @@
+"""MI355X-native 3D FFT convolution (drop-in for libFourierConvolutionCUDALib).
+
+Two Python faces over the same C ABI (``include/*.h``):
+
+* a host mirror of the reference interface (``src/convolution3Dfft.h``):
+  ``convolution3DfftCUDAInPlace(im, imDim, kernel, kernelDim, devCUDA)`` and
+  friends, on numpy float32 buffers, with the reference's argument meaning
+  (dims ``{x, y, z}``, x contiguous).  Failures raise :class:`FCError`
+  (the reference prints and calls ``exit``, ``src/book.h:22-30``);
+* a device API on torch CUDA tensors (``convolve3d``), asynchronous on a
+  stream, used by the benchmark, the autograd wrapper and the batched /
+  deconvolution drivers.
+
+Volumes are C-order ``[nz, ny, nx]`` arrays.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from ._lib import FCError, check, ints, lib
+
+__all__ = [
+    "FCError", "lib",
+    "convolution3DfftCUDAInPlace", "convolution3DfftCUDA", "convolution3DfftCUDA_test",
+    "selectDeviceWithHighestComputeCapability", "getCUDAcomputeCapabilityMajorVersion",
+    "getCUDAcomputeCapabilityMinorVersion", "getNumDevicesCUDA", "getNameDeviceCUDA",
+    "getMemDeviceCUDA", "cuda_version", "gpu_mem_needed_mb",
+    "convolve", "convolve3d", "workspace_bytes", "radix_plan",
+    "CONVOLVE", "CORRELATE",
+]
+
+CONVOLVE = 0
+CORRELATE = 1
+
+_fp = ctypes.POINTER(ctypes.c_float)
+
+
+def _f32_buffer(a, name):
+    if not isinstance(a, np.ndarray) or a.dtype != np.float32 or not a.flags.c_contiguous:
+        raise TypeError(f"{name} must be a C-contiguous float32 numpy array")
+    return a.ctypes.data_as(_fp)
+
+
+def _raise_if_failed():
+    code = lib().fcamd_last_error()
+    if code != 0:
+        check(code)
+
+
+# ---------------------------------------------------------------------------
+# reference interface (host pointers)
+# ---------------------------------------------------------------------------
+def convolution3DfftCUDAInPlace(im, imDim, kernel, kernelDim, devCUDA=0):
+    """src/convolution3Dfft.h:56 -- overwrite ``im`` with its circular
+    convolution by ``kernel`` (zero-padded, centred at floor(k/2))."""
+    if int(np.prod(imDim)) != im.size or int(np.prod(kernelDim)) != kernel.size:
+        raise ValueError("dims do not match buffer sizes")
+    lib().convolution3DfftCUDAInPlace(_f32_buffer(im, "im"), ints(imDim),
+                                      _f32_buffer(kernel, "kernel"), ints(kernelDim),
+                                      int(devCUDA))
+    _raise_if_failed()
+
+
+def _take_result(ptr, n):
+    if not ptr:
+        _raise_if_failed()
+        raise FCError(6, "no result returned")
+    try:
+        return np.ctypeslib.as_array(ptr, shape=(n,)).copy()
+    finally:
+        lib().fcamd_free_result(ptr)
+
+
+def convolution3DfftCUDA(im, imDim, kernel, kernelDim, devCUDA=0):
+    """src/convolution3Dfft.h:41-45 -- dims z-fastest (imDim[2] contiguous);
+    returns a new flat float32 array, ``im`` untouched."""
+    if int(np.prod(imDim)) != im.size or int(np.prod(kernelDim)) != kernel.size:
+        raise ValueError("dims do not match buffer sizes")
+    ptr = lib().convolution3DfftCUDA(_f32_buffer(im, "im"), ints(imDim),
+                                     _f32_buffer(kernel, "kernel"), ints(kernelDim),
+                                     int(devCUDA))
+    return _take_result(ptr, im.size)
+
+
+def convolution3DfftCUDA_test(im, imDim, kernel, devCUDA=0):
+    """src/convolution3Dfft.h:29-32 -- image-sized, unshifted kernel."""
+    if int(np.prod(imDim)) != im.size or kernel.size != im.size:
+        raise ValueError("dims do not match buffer sizes")
+    ptr = lib().convolution3DfftCUDA_test(_f32_buffer(im, "im"), ints(imDim),
+                                          _f32_buffer(kernel, "kernel"), int(devCUDA))
+    return _take_result(ptr, im.size)
+
+
+def selectDeviceWithHighestComputeCapability():
+    return lib().selectDeviceWithHighestComputeCapability()
+
+
+def getCUDAcomputeCapabilityMajorVersion(dev):
+    return lib().getCUDAcomputeCapabilityMajorVersion(int(dev))
+
+
+def getCUDAcomputeCapabilityMinorVersion(dev):
+    return lib().getCUDAcomputeCapabilityMinorVersion(int(dev))
+
+
+def getNumDevicesCUDA():
+    return lib().getNumDevicesCUDA()
+
+
+def getNameDeviceCUDA(dev):
+    buf = ctypes.create_string_buffer(256)
+    lib().getNameDeviceCUDA(int(dev), buf)
+    return buf.value.decode(errors="replace")
+
+
+def getMemDeviceCUDA(dev):
+    return lib().getMemDeviceCUDA(int(dev))
+
+
+def cuda_version():
+    return lib().cuda_version()
+
+
+def gpu_mem_needed_mb(shape):
+    return lib().gpu_mem_needed_mb(ints(shape), len(shape))
+
+
+# ---------------------------------------------------------------------------
+# conveniences
+# ---------------------------------------------------------------------------
+def convolve(image, kernel, op=CONVOLVE, dev=0):
+    """Circular convolution of a 3D numpy volume (host path); returns a new
+    float32 array.  ``op=CORRELATE`` gives the adjoint (flipped kernel)."""
+    im = np.array(image, dtype=np.float32, order="C", copy=True)
+    k = np.ascontiguousarray(kernel, dtype=np.float32)
+    if im.ndim != 3 or k.ndim != 3:
+        raise ValueError("image and kernel must be 3D")
+    check(lib().fcamd_convolve3d_host(_f32_buffer(im, "image"), ints(im.shape[::-1]),
+                                      _f32_buffer(k, "kernel"), ints(k.shape[::-1]),
+                                      int(op), int(dev)))
+    return im
+
+
+def workspace_bytes(image_shape, kernel_shape):
+    """Device workspace for one convolution of C-order shapes (z, y, x)."""
+    n = lib().fcamd_workspace_bytes(ints(tuple(image_shape)[::-1]), ints(tuple(kernel_shape)[::-1]))
+    if n < 0:
+        _raise_if_failed()
+    return int(n)
+
+
+def radix_plan(length):
+    buf = (ctypes.c_int * 64)()
+    n = lib().fcamd_radix_plan(int(length), buf, 64)
+    return [buf[i] for i in range(n)]
+
+
+def _stream_handle(stream):
+    if stream is None:
+        import torch
+        return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    if isinstance(stream, int):
+        return ctypes.c_void_p(stream)
+    return ctypes.c_void_p(stream.cuda_stream)
+
+
+def convolve3d(image, kernel, op=CONVOLVE, out=None, workspace=None, stream=None):
+    """Device-resident convolution of torch CUDA float32 tensors.
+
+    image [nz, ny, nx], kernel [kz, ky, kx] (contiguous, same device);
+    ``out`` may be ``image`` (in place).  Enqueued on ``stream`` (default:
+    torch's current stream); returns ``out``."""
+    import torch
+    if image.dtype != torch.float32 or kernel.dtype != torch.float32:
+        raise TypeError("float32 tensors required")
+    if not (image.is_cuda and kernel.is_cuda):
+        raise TypeError("CUDA tensors required (there is no CPU path)")
+    if image.dim() != 3 or kernel.dim() != 3:
+        raise ValueError("image and kernel must be 3D")
+    image = image.contiguous()
+    kernel = kernel.contiguous()
+    if out is None:
+        out = torch.empty_like(image)
+    elif out.shape != image.shape or not out.is_contiguous():
+        raise ValueError("out must be a contiguous tensor shaped like image")
+    wptr, wbytes = None, 0
+    if workspace is not None:
+        wptr, wbytes = workspace.data_ptr(), workspace.numel() * workspace.element_size()
+    with torch.cuda.device(image.device):
+        check(lib().fcamd_convolve3d_device(
+            out.data_ptr(), image.data_ptr(), ints(image.shape[::-1]),
+            kernel.data_ptr(), ints(kernel.shape[::-1]), int(op),
+            wptr, wbytes, _stream_handle(stream)))
+    return out

@@ -1,0 +1,85 @@
+"""Loading of libFourierConvolutionCUDALib.so (the MI355X build) via ctypes.
+
+The library is built in-tree by ``make -C fourierconvolutioncudalib_amd/csrc``
+(``__graft_entry__.build()`` does it).  Loading fails loudly when it is
+missing: there is no CPU fallback for any entry point.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libFourierConvolutionCUDALib.so")
+CSRC = os.path.join(_HERE, "csrc")
+
+_lib = None
+
+_i = ctypes.c_int
+_ip = ctypes.POINTER(ctypes.c_int)
+_fp = ctypes.POINTER(ctypes.c_float)
+_vp = ctypes.c_void_p
+_ll = ctypes.c_longlong
+
+# name -> (restype, argtypes); every symbol include/*.h declares
+SIGNATURES = {
+    # include/convolution3Dfft.h (reference ABI)
+    "convolution3DfftCUDA_test": (_fp, [_fp, _ip, _fp, _i]),
+    "convolution3DfftCUDA": (_fp, [_fp, _ip, _fp, _ip, _i]),
+    "convolution3DfftCUDAInPlace": (None, [_fp, _ip, _fp, _ip, _i]),
+    "selectDeviceWithHighestComputeCapability": (_i, []),
+    "getCUDAcomputeCapabilityMinorVersion": (_i, [_i]),
+    "getCUDAcomputeCapabilityMajorVersion": (_i, [_i]),
+    "getNumDevicesCUDA": (_i, []),
+    "getNameDeviceCUDA": (None, [_i, ctypes.c_char_p]),
+    "getMemDeviceCUDA": (_ll, [_i]),
+    "cuda_version": (_i, []),
+    "gpu_mem_needed_mb": (_i, [_ip, _i]),
+    # include/fourierconvolution_amd.h (additive ABI)
+    "fcamd_last_error": (_i, []),
+    "fcamd_last_error_message": (ctypes.c_char_p, []),
+    "fcamd_free_result": (None, [_fp]),
+    "fcamd_convolve3d_host": (_i, [_fp, _ip, _fp, _ip, _i, _i]),
+    "fcamd_workspace_bytes": (_ll, [_ip, _ip]),
+    "fcamd_convolve3d_device": (_i, [_vp, _vp, _ip, _vp, _ip, _i, _vp, _ll, _vp]),
+    "fcamd_radix_plan": (_i, [_i, _ip, _i]),
+    "fcamd_has_static_kernel": (_i, [_i, _i]),
+    "fcamd_modulate_device": (_i, [_vp, _vp, _ll, ctypes.c_float, _i, _vp]),
+    "fcamd_stream_copy_device": (_i, [_vp, _vp, _ll, _vp]),
+}
+
+
+class FCError(RuntimeError):
+    """A library call failed; ``code`` is the fcamd status code."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(f"[fcamd error {code}] {message}")
+        self.code = code
+
+
+def lib():
+    """The loaded library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with "
+                f"`make -C {CSRC}` (or __graft_entry__.build())")
+        handle = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+def check(rc: int) -> None:
+    if rc != 0:
+        msg = lib().fcamd_last_error_message()
+        raise FCError(rc, msg.decode() if msg else "unknown error")
+
+
+def ints(values):
+    values = [int(v) for v in values]
+    return (ctypes.c_int * len(values))(*values)

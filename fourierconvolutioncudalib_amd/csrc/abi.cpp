@@ -1,0 +1,287 @@
+// abi.cpp -- the exported C ABI: the reference's eleven symbols
+// (src/convolution3Dfft.h:29-84) and the additive fcamd_* entry points.
+// No exception or exit() crosses this boundary.
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/convolution3Dfft.h"
+#include "../../include/fourierconvolution_amd.h"
+#include "conv_driver.hpp"
+#include "static_kernels.hpp"
+
+using namespace fcamd;
+
+namespace fcamd {
+
+namespace {
+thread_local int t_code = 0;
+thread_local std::string t_msg;
+}  // namespace
+
+void set_error(int code, const std::string& msg) {
+  t_code = code;
+  t_msg = msg;
+}
+void clear_error() {
+  t_code = 0;
+  t_msg.clear();
+}
+int last_error_code() { return t_code; }
+const char* last_error_message() { return t_msg.c_str(); }
+
+}  // namespace fcamd
+
+namespace {
+
+template <class F>
+int guarded(F&& f) {
+  clear_error();
+  try {
+    f();
+    return kOk;
+  } catch (const Error& e) {
+    set_error(e.code, e.msg);
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    set_error(kOutOfMemory, "host allocation failed");
+    return kOutOfMemory;
+  } catch (...) {
+    set_error(kInternal, "unexpected internal error");
+    return kInternal;
+  }
+}
+
+ConvShape shape_from(const int* imDim, const int* kernelDim) {
+  FC_REQUIRE(imDim != nullptr && kernelDim != nullptr, kInvalidArgument, "null dimension array");
+  ConvShape s{};
+  s.nx = imDim[0];
+  s.ny = imDim[1];
+  s.nz = imDim[2];
+  s.kx = kernelDim[0];
+  s.ky = kernelDim[1];
+  s.kz = kernelDim[2];
+  s.cx = s.kx / 2;
+  s.cy = s.ky / 2;
+  s.cz = s.kz / 2;
+  validate_shape(s);
+  return s;
+}
+
+Op op_from(int op) {
+  FC_REQUIRE(op == FCAMD_CONVOLVE || op == FCAMD_CORRELATE, kInvalidArgument, "unknown op");
+  return op == FCAMD_CORRELATE ? Op::kCorrelate : Op::kConvolve;
+}
+
+// Host-pointer convolution on device `dev`: H2D, passes, D2H into `out`.
+// `out` may alias `im`.  Serialised per device (pooled buffers).
+void host_convolution(const ConvShape& s, const float* im, const float* kernel, float* out,
+                      Op op, int dev) {
+  FC_REQUIRE(im != nullptr && kernel != nullptr && out != nullptr, kInvalidArgument,
+             "null data pointer");
+  DeviceGuard guard(dev);
+  DevicePool& pool = device_pool(dev);
+  std::lock_guard<std::mutex> lk(pool.mu);
+  if (!pool.stream) FC_HIP_CHECK(hipStreamCreateWithFlags(&pool.stream, hipStreamNonBlocking));
+  const size_t nimg = sizeof(float) * (size_t)s.voxels();
+  const size_t nker = sizeof(float) * (size_t)s.kx * s.ky * s.kz;
+  const size_t nws = workspace_layout(s).total();
+  void* dimg = pool_reserve(reinterpret_cast<void**>(&pool.img), &pool.img_bytes, nimg);
+  void* dker = pool_reserve(reinterpret_cast<void**>(&pool.ker), &pool.ker_bytes, nker);
+  void* ws = pool_reserve(&pool.ws, &pool.ws_bytes, nws);
+  FC_HIP_CHECK(hipMemcpyAsync(dimg, im, nimg, hipMemcpyHostToDevice, pool.stream));
+  FC_HIP_CHECK(hipMemcpyAsync(dker, kernel, nker, hipMemcpyHostToDevice, pool.stream));
+  enqueue_convolution(s, static_cast<float*>(dimg), static_cast<float*>(dimg),
+                      static_cast<float*>(dker), op, ws, pool.stream);
+  FC_HIP_CHECK(hipMemcpyAsync(out, dimg, nimg, hipMemcpyDeviceToHost, pool.stream));
+  FC_HIP_CHECK(hipStreamSynchronize(pool.stream));
+}
+
+}  // namespace
+
+// ===========================================================================
+// reference symbols
+// ===========================================================================
+extern "C" {
+
+// src/convolution3Dfft.h:56, src/convolution3Dfft.cu:399-578
+void convolution3DfftCUDAInPlace(imageType* im, int* imDim, imageType* kernel, int* kernelDim,
+                                 int devCUDA) {
+  (void)guarded([&] {
+    ConvShape s = shape_from(imDim, kernelDim);
+    host_convolution(s, im, kernel, im, Op::kConvolve, devCUDA);
+  });
+}
+
+// src/convolution3Dfft.h:41-45, src/convolution3Dfft.cu:299-391.  Dims are
+// z-fastest (imDim[2] contiguous), so {imDim[2], imDim[1], imDim[0]} is the
+// {x, y, z} of the in-place entry point.  The reference computes this
+// in-place on an unpadded R2C buffer, valid only under the long-removed
+// cuFFT native layout (:298,358-361); this build returns the convolution.
+imageType* convolution3DfftCUDA(imageType* im, int* imDim, imageType* kernel, int* kernelDim,
+                                int devCUDA) {
+  float* result = nullptr;
+  int rc = guarded([&] {
+    FC_REQUIRE(imDim != nullptr && kernelDim != nullptr, kInvalidArgument,
+               "null dimension array");
+    const int d[3] = {imDim[2], imDim[1], imDim[0]};
+    const int k[3] = {kernelDim[2], kernelDim[1], kernelDim[0]};
+    ConvShape s = shape_from(d, k);
+    result = new float[(size_t)s.voxels()];
+    host_convolution(s, im, kernel, result, Op::kConvolve, devCUDA);
+  });
+  if (rc != kOk) {
+    delete[] result;
+    return nullptr;
+  }
+  return result;
+}
+
+// src/convolution3Dfft.h:29-32, src/convolution3Dfft.cu:216-292: the kernel is
+// image-sized and used without the fftShift (its tap 0 sits at index 0).
+imageType* convolution3DfftCUDA_test(imageType* im, int* imDim, imageType* kernel, int devCUDA) {
+  float* result = nullptr;
+  int rc = guarded([&] {
+    FC_REQUIRE(imDim != nullptr, kInvalidArgument, "null dimension array");
+    ConvShape s{};
+    s.nx = imDim[2];
+    s.ny = imDim[1];
+    s.nz = imDim[0];
+    s.kx = s.nx;
+    s.ky = s.ny;
+    s.kz = s.nz;
+    s.cx = s.cy = s.cz = 0;
+    validate_shape(s);
+    result = new float[(size_t)s.voxels()];
+    host_convolution(s, im, kernel, result, Op::kConvolve, devCUDA);
+  });
+  if (rc != kOk) {
+    delete[] result;
+    return nullptr;
+  }
+  return result;
+}
+
+// src/convolution3Dfft.cu:581-584
+int cuda_version() { return HIP_VERSION; }
+
+// src/convolution3Dfft.cu:587-603 (cufftEstimate*d): the half-spectrum
+// workspace of an R2C transform of `shape` (last entry contiguous), in MiB.
+int gpu_mem_needed_mb(int* shape, int len) {
+  if (shape == nullptr || len < 1 || len > 3) return -1;
+  long long outer = 1;
+  for (int i = 0; i < len - 1; ++i) {
+    if (shape[i] < 1) return -1;
+    outer *= shape[i];
+  }
+  if (shape[len - 1] < 1) return -1;
+  const long long bytes = outer * padded_bins(shape[len - 1]) * (long long)sizeof(float2);
+  return (int)(bytes / (1 << 20));
+}
+
+// ===========================================================================
+// src/standardCUDAfunctions.cu:13-71 on HIP
+// ===========================================================================
+int getNumDevicesCUDA() {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+int selectDeviceWithHighestComputeCapability() {
+  const int n = getNumDevicesCUDA();
+  int best = -1, best_cc = 0;
+  for (int d = 0; d < n; ++d) {
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, d) != hipSuccess) continue;
+    const int cc = 10 * p.major + p.minor;
+    if (cc > best_cc) {
+      best_cc = cc;
+      best = d;
+    }
+  }
+  return best;
+}
+
+int getCUDAcomputeCapabilityMajorVersion(int devCUDA) {
+  hipDeviceProp_t p;
+  if (hipGetDeviceProperties(&p, devCUDA) != hipSuccess) return -1;
+  return p.major;
+}
+
+int getCUDAcomputeCapabilityMinorVersion(int devCUDA) {
+  hipDeviceProp_t p;
+  if (hipGetDeviceProperties(&p, devCUDA) != hipSuccess) return -1;
+  return p.minor;
+}
+
+void getNameDeviceCUDA(int devCUDA, char* name) {
+  if (name == nullptr) return;
+  char buf[256];
+  std::memset(buf, 0, sizeof(buf));
+  hipDeviceProp_t p;
+  if (hipGetDeviceProperties(&p, devCUDA) == hipSuccess)
+    std::strncpy(buf, p.name, sizeof(buf) - 1);
+  std::memcpy(name, buf, sizeof(buf));
+}
+
+long long int getMemDeviceCUDA(int devCUDA) {
+  hipDeviceProp_t p;
+  if (hipGetDeviceProperties(&p, devCUDA) != hipSuccess) return -1;
+  return (long long)p.totalGlobalMem;
+}
+
+// ===========================================================================
+// additive entry points
+// ===========================================================================
+int fcamd_last_error(void) { return last_error_code(); }
+const char* fcamd_last_error_message(void) { return last_error_message(); }
+void fcamd_free_result(float* p) { delete[] p; }
+
+int fcamd_convolve3d_host(float* im, const int* imDim, const float* kernel, const int* kernelDim,
+                          int op, int devCUDA) {
+  return guarded([&] {
+    ConvShape s = shape_from(imDim, kernelDim);
+    host_convolution(s, im, kernel, im, op_from(op), devCUDA);
+  });
+}
+
+long long fcamd_workspace_bytes(const int* imDim, const int* kernelDim) {
+  long long r = -1;
+  (void)guarded([&] { r = (long long)workspace_layout(shape_from(imDim, kernelDim)).total(); });
+  return r;
+}
+
+int fcamd_convolve3d_device(float* d_out, const float* d_image, const int* imDim,
+                            const float* d_kernel, const int* kernelDim, int op, void* d_workspace,
+                            long long workspace_bytes, void* stream) {
+  return guarded([&] {
+    FC_REQUIRE(d_out && d_image && d_kernel, kInvalidArgument, "null device pointer");
+    ConvShape s = shape_from(imDim, kernelDim);
+    const Op o = op_from(op);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const size_t need = workspace_layout(s).total();
+    if (d_workspace) {
+      FC_REQUIRE(workspace_bytes >= (long long)need, kInvalidArgument, "workspace too small");
+      enqueue_convolution(s, d_out, d_image, d_kernel, o, d_workspace, st);
+    } else {
+      int dev = 0;
+      FC_HIP_CHECK(hipGetDevice(&dev));
+      DevicePool& pool = device_pool(dev);
+      std::lock_guard<std::mutex> lk(pool.mu);
+      void* ws = pool_reserve(&pool.ws, &pool.ws_bytes, need);
+      enqueue_convolution(s, d_out, d_image, d_kernel, o, ws, st);
+      // the pooled workspace is reused by the next call: finish before unlocking
+      FC_HIP_CHECK(hipStreamSynchronize(st));
+    }
+  });
+}
+
+int fcamd_radix_plan(int length, int* radices, int cap) {
+  if (length < 1) return -1;
+  std::vector<int> r = radix_plan(length);
+  for (int i = 0; i < (int)r.size() && i < cap; ++i) radices[i] = r[i];
+  return (int)r.size();
+}
+
+}  // extern "C"

@@ -1,0 +1,290 @@
+// conv_driver.cpp -- host driver of the 3D FFT convolution.
+//
+// Pass sequence for one convolution (reference: the body of
+// convolution3DfftCUDAInPlace, src/convolution3Dfft.cu:399-578, which pads
+// the kernel with fftShiftKernel, runs cuFFT R2C on image and kernel,
+// modulateAndNormalize_kernel and cuFFT C2R):
+//   K1  kernel rows   x-R2C of the ky*kz compact rows, taps placed at
+//                     (x - cx) mod nx                 -> Kx  [kz][ky][nxp]
+//   K2  kernel planes y-FFT of Kx columns, taps placed at (y - cy) mod ny
+//                                                     -> Kxy [kz][ny][nxp]
+//   P1  image x-R2C                                   -> S   [nz][ny][nxp]
+//   P2  y-FFT of S in place
+//   P3  z pass: FFTz(S) * FFTz(Kxy placed at (z - cz) mod nz) / N, IFFTz
+//   P4  inverse y-FFT of S in place
+//   P5  x-C2R of S into the output volume
+// The kernel is never materialised at image size.
+#include "conv_driver.hpp"
+
+#include <cmath>
+#include <map>
+#include <memory>
+#include <string>
+
+#include "static_kernels.hpp"
+
+namespace fcamd {
+
+// ---------------------------------------------------------------------------
+// plans and twiddle tables
+// ---------------------------------------------------------------------------
+std::vector<int> radix_plan(int L) {
+  std::vector<int> r;
+  int n = L;
+  while (n % 8 == 0) { r.push_back(8); n /= 8; }
+  if (n % 4 == 0) { r.push_back(4); n /= 4; }
+  if (n % 2 == 0) { r.push_back(2); n /= 2; }
+  for (int p : {3, 5, 7})
+    while (n % p == 0) { r.push_back(p); n /= p; }
+  for (int p = 11; (long)p * p <= n; p += 2)
+    while (n % p == 0) { r.push_back(p); n /= p; }
+  if (n > 1) r.push_back(n);
+  return r;
+}
+
+namespace {
+
+std::mutex g_tw_mu;
+std::map<std::pair<int, int>, float2*> g_tw;  // (device, L) -> table
+
+float2* twiddle_table(int dev, int L) {
+  std::lock_guard<std::mutex> lk(g_tw_mu);
+  auto key = std::make_pair(dev, L);
+  auto it = g_tw.find(key);
+  if (it != g_tw.end()) return it->second;
+  std::vector<float2> h(L);
+  for (int j = 0; j < L; ++j) {
+    // exact integer j < L keeps the angle in [0, 2pi); FP64 then one rounding
+    const double a = -2.0 * M_PI * (double)j / (double)L;
+    h[j] = make_float2((float)std::cos(a), (float)std::sin(a));
+  }
+  float2* d = nullptr;
+  FC_HIP_CHECK(hipMalloc(&d, sizeof(float2) * (size_t)L));
+  FC_HIP_CHECK(hipMemcpy(d, h.data(), sizeof(float2) * (size_t)L, hipMemcpyHostToDevice));
+  g_tw[key] = d;
+  return d;
+}
+
+int current_device() {
+  int dev = 0;
+  FC_HIP_CHECK(hipGetDevice(&dev));
+  return dev;
+}
+
+std::once_flag g_attr_once;
+
+}  // namespace
+
+FftDesc fft_desc(int L) {
+  FftDesc d{};
+  d.L = L;
+  std::vector<int> r = radix_plan(L);
+  FC_REQUIRE((int)r.size() <= kMaxStages, kUnsupportedSize, "FFT plan too deep");
+  d.nst = (int)r.size();
+  for (int i = 0; i < d.nst; ++i) d.rad[i] = r[i];
+  d.tw = twiddle_table(current_device(), L);
+  return d;
+}
+
+// ---------------------------------------------------------------------------
+// generic tile sizes (bytes of LDS per workgroup)
+// ---------------------------------------------------------------------------
+namespace {
+constexpr size_t kLdsMax = 160 * 1024;
+constexpr size_t kLdsTarget = 64 * 1024;
+
+int x_pairs_per_block(int L) {
+  const size_t per = 2 * sizeof(float2) * (size_t)(L + 1);
+  int p = (int)(32768 / per);
+  if (p < 1) p = 1;
+  if (p > 8) p = 8;
+  return p;
+}
+
+int col_tile(int L, int nbuf) {
+  const size_t per = (size_t)nbuf * sizeof(float2) * (size_t)L;
+  int tx = 16;
+  while (tx > 1 && per * tx > kLdsTarget) tx /= 2;
+  return tx;
+}
+}  // namespace
+
+void validate_shape(const ConvShape& s) {
+  FC_REQUIRE(s.nx >= 1 && s.ny >= 1 && s.nz >= 1, kInvalidArgument,
+             "image dimensions must be >= 1");
+  FC_REQUIRE(s.kx >= 1 && s.ky >= 1 && s.kz >= 1, kInvalidArgument,
+             "kernel dimensions must be >= 1");
+  FC_REQUIRE(s.kx <= s.nx && s.ky <= s.ny && s.kz <= s.nz, kInvalidArgument,
+             "kernel extent exceeds image extent on some axis");
+  FC_REQUIRE(s.cx >= 0 && s.cx < s.kx && s.cy >= 0 && s.cy < s.ky && s.cz >= 0 && s.cz < s.kz,
+             kInvalidArgument, "kernel centre outside the kernel");
+  // LDS capacity of the runtime-length kernels
+  FC_REQUIRE(2 * sizeof(float2) * (size_t)(s.nx + 1) <= kLdsMax, kUnsupportedSize,
+             "x extent too large for a single-tile FFT");
+  FC_REQUIRE(2 * sizeof(float2) * (size_t)s.ny <= kLdsMax, kUnsupportedSize,
+             "y extent too large for a single-tile FFT");
+  FC_REQUIRE(3 * sizeof(float2) * (size_t)s.nz <= kLdsMax, kUnsupportedSize,
+             "z extent too large for a single-tile FFT");
+  FC_REQUIRE(s.ny <= 65536 && s.nz <= 65536, kUnsupportedSize, "y/z extent exceeds grid limits");
+}
+
+static size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+
+WorkspaceLayout workspace_layout(const ConvShape& s) {
+  const size_t nxp = (size_t)padded_bins(s.nx);
+  WorkspaceLayout w;
+  w.spectrum = align256(sizeof(float2) * nxp * s.ny * s.nz);
+  w.krows = align256(sizeof(float2) * nxp * s.ky * s.kz);
+  w.kplanes = align256(sizeof(float2) * nxp * s.ny * s.kz);
+  return w;
+}
+
+void enqueue_convolution(const ConvShape& s, float* d_out, const float* d_image,
+                         const float* d_kernel, Op op, void* ws, hipStream_t st) {
+  std::call_once(g_attr_once, [] {
+    generic_kernels_allow_large_lds();
+    static_kernels_allow_large_lds();
+  });
+  const WorkspaceLayout wl = workspace_layout(s);
+  char* base = static_cast<char*>(ws);
+  float2* S = reinterpret_cast<float2*>(base);
+  float2* Kx = reinterpret_cast<float2*>(base + wl.spectrum);
+  float2* Kxy = reinterpret_cast<float2*>(base + wl.spectrum + wl.krows);
+  const int nb = half_bins(s.nx);
+  const long nxp = padded_bins(s.nx);
+  const FftDesc fx = fft_desc(s.nx), fy = fft_desc(s.ny), fz = fft_desc(s.nz);
+
+  // K1: kernel rows
+  {
+    XPassArgs a{};
+    a.fft = fx;
+    a.nrows = (long)s.ky * s.kz;
+    a.nbins = nb;
+    a.pairs_per_block = x_pairs_per_block(s.nx);
+    a.src = d_kernel;
+    a.dst = Kx;
+    a.src_row = s.kx;
+    a.dst_row = nxp;
+    a.kshift = 1;
+    a.klen = s.kx;
+    a.kcenter = s.cx;
+    launch_gk_xfwd(a, (a.nrows + 1) / 2, st);
+  }
+  // K2: kernel planes
+  {
+    ColArgs a{};
+    a.fft = fy;
+    a.tx = col_tile(s.ny, 2);
+    a.nbins = nb;
+    a.src = Kx;
+    a.dst = Kxy;
+    a.src_outer = (long)s.ky * nxp;
+    a.src_elem = nxp;
+    a.dst_outer = (long)s.ny * nxp;
+    a.dst_elem = nxp;
+    a.nvalid = s.ky;
+    a.center = s.cy;
+    launch_gk_col(a, -1, s.kz, st);
+  }
+  // P1: image rows
+  const long nrows = (long)s.ny * s.nz;
+  if (!static_xfwd(s, fx, d_image, S, st)) {
+    XPassArgs a{};
+    a.fft = fx;
+    a.nrows = nrows;
+    a.nbins = nb;
+    a.pairs_per_block = x_pairs_per_block(s.nx);
+    a.src = d_image;
+    a.dst = S;
+    a.src_row = s.nx;
+    a.dst_row = nxp;
+    launch_gk_xfwd(a, (nrows + 1) / 2, st);
+  }
+  // P2 / P4: y passes in place
+  ColArgs ya{};
+  ya.fft = fy;
+  ya.tx = col_tile(s.ny, 2);
+  ya.nbins = nb;
+  ya.src = S;
+  ya.dst = S;
+  ya.src_outer = ya.dst_outer = (long)s.ny * nxp;
+  ya.src_elem = ya.dst_elem = nxp;
+  ya.nvalid = s.ny;
+  ya.center = 0;
+  if (s.ny > 1 && !static_ycol(s, fy, S, -1, st)) launch_gk_col(ya, -1, s.nz, st);
+  // P3: fused z pass
+  {
+    ZArgs a{};
+    a.fft = fz;
+    a.tx = col_tile(s.nz, 3);
+    a.nbins = nb;
+    a.s = S;
+    a.s_row = nxp;
+    a.s_plane = (long)s.ny * nxp;
+    a.k = Kxy;
+    a.k_row = nxp;
+    a.k_plane = (long)s.ny * nxp;
+    a.nvalid = s.kz;
+    a.center = s.cz;
+    a.conj = op == Op::kCorrelate ? 1 : 0;
+    a.scale = (float)(1.0 / (double)s.voxels());
+    if (!static_zfused(s, a, st)) launch_gk_zfused(a, s.ny, st);
+  }
+  if (s.ny > 1 && !static_ycol(s, fy, S, +1, st)) launch_gk_col(ya, +1, s.nz, st);
+  // P5: back to real rows
+  if (!static_xinv(s, fx, S, d_out, st)) {
+    XPassArgs a{};
+    a.fft = fx;
+    a.nrows = nrows;
+    a.nbins = nb;
+    a.pairs_per_block = x_pairs_per_block(s.nx);
+    a.csrc = S;
+    a.rdst = d_out;
+    a.src_row = nxp;
+    a.dst_row = s.nx;
+    launch_gk_xinv(a, (nrows + 1) / 2, st);
+  }
+  FC_HIP_CHECK(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// pools and device guard
+// ---------------------------------------------------------------------------
+DevicePool& device_pool(int dev) {
+  static std::mutex mu;
+  static std::map<int, std::unique_ptr<DevicePool>> pools;
+  std::lock_guard<std::mutex> lk(mu);
+  auto& p = pools[dev];
+  if (!p) p.reset(new DevicePool());
+  return *p;
+}
+
+void* pool_reserve(void** p, size_t* cap, size_t bytes) {
+  if (bytes == 0) bytes = 256;
+  if (*p && *cap >= bytes) return *p;
+  if (*p) {
+    FC_HIP_CHECK(hipDeviceSynchronize());
+    FC_HIP_CHECK(hipFree(*p));
+    *p = nullptr;
+    *cap = 0;
+  }
+  FC_HIP_CHECK(hipMalloc(p, bytes));
+  *cap = bytes;
+  return *p;
+}
+
+DeviceGuard::DeviceGuard(int dev) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
+    throw Error{kNoDevice, "no HIP device available"};
+  FC_REQUIRE(dev >= 0 && dev < n, kInvalidArgument, "device index out of range");
+  FC_HIP_CHECK(hipGetDevice(&prev_));
+  if (prev_ != dev) FC_HIP_CHECK(hipSetDevice(dev));
+}
+
+DeviceGuard::~DeviceGuard() {
+  int cur = -1;
+  if (prev_ >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev_) (void)hipSetDevice(prev_);
+}
+
+}  // namespace fcamd

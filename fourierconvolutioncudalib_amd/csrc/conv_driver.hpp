@@ -1,0 +1,75 @@
+// conv_driver.hpp -- host driver of the 3D FFT convolution (plans, tables,
+// workspaces, pass sequence).  All sizes use the reference's {x, y, z}
+// naming with x the contiguous axis (src/convolution3Dfft.cu:394-421).
+#pragma once
+
+#include <mutex>
+#include <vector>
+
+#include "fc_internal.hpp"
+#include "kernels.hpp"
+
+namespace fcamd {
+
+// One convolution problem.  kc* are the kernel centres (where kernel tap
+// (0,0,0) is taken from): k/2 for the drop-in entry points, 0 for the
+// pre-shifted image-sized kernel of convolution3DfftCUDA_test.
+struct ConvShape {
+  int nx, ny, nz;
+  int kx, ky, kz;
+  int cx, cy, cz;
+  long long voxels() const { return (long long)nx * ny * nz; }
+};
+
+// Workspace carve-up (bytes, each region 256-byte aligned).
+struct WorkspaceLayout {
+  size_t spectrum;  // S   [nz][ny][nxp] float2
+  size_t krows;     // Kx  [kz][ky][nxp] float2
+  size_t kplanes;   // Kxy [kz][ny][nxp] float2
+  size_t total() const { return spectrum + krows + kplanes; }
+};
+
+WorkspaceLayout workspace_layout(const ConvShape& s);
+
+// Validates the shape; throws Error on failure.
+void validate_shape(const ConvShape& s);
+
+// FFT description for length L on the current device (tables cached).
+FftDesc fft_desc(int L);
+
+// Radix plan used for length L.
+std::vector<int> radix_plan(int L);
+
+enum class Op : int { kConvolve = 0, kCorrelate = 1 };
+
+// Enqueues the full convolution on `stream`: d_out may alias d_image.
+// `ws` must hold workspace_layout(s).total() bytes.
+void enqueue_convolution(const ConvShape& s, float* d_out, const float* d_image,
+                         const float* d_kernel, Op op, void* ws, hipStream_t stream);
+
+// Per-device pooled resources for callers that pass no workspace.
+struct DevicePool {
+  std::mutex mu;
+  void* ws = nullptr;
+  size_t ws_bytes = 0;
+  float* img = nullptr;
+  size_t img_bytes = 0;
+  float* ker = nullptr;
+  size_t ker_bytes = 0;
+  hipStream_t stream = nullptr;
+};
+
+DevicePool& device_pool(int dev);
+void* pool_reserve(void** p, size_t* cap, size_t bytes);
+
+// RAII: makes `dev` current and restores the caller's device on exit.
+class DeviceGuard {
+ public:
+  explicit DeviceGuard(int dev);
+  ~DeviceGuard();
+
+ private:
+  int prev_ = -1;
+};
+
+}  // namespace fcamd

@@ -1,0 +1,61 @@
+// kernels.hpp -- argument blocks and launchers of the pass kernels.
+#pragma once
+
+#include "fc_internal.hpp"
+
+namespace fcamd {
+
+// x pass (rows along the contiguous axis).  Forward reads real rows (`src`)
+// and writes half-spectrum rows (`dst`); inverse reads `csrc`, writes `rdst`.
+struct XPassArgs {
+  FftDesc fft;          // length nx
+  long nrows;           // rows to transform
+  int nbins;            // nx/2 + 1
+  int pairs_per_block;  // row pairs per workgroup
+  const float* src;     // forward input
+  float2* dst;          // forward output
+  const float2* csrc;   // inverse input
+  float* rdst;          // inverse output
+  long src_row;         // element stride between input rows
+  long dst_row;         // element stride between output rows
+  int kshift;           // forward only: rows are compact kernel rows
+  int klen;             // compact row length (kx)
+  int kcenter;          // kx / 2
+};
+
+// column pass along y or z over a dense half spectrum (or from the compact
+// kernel rows into the kernel planes).
+struct ColArgs {
+  FftDesc fft;   // column length
+  int tx;        // bins per tile
+  int nbins;     // valid bins per row (nx/2 + 1)
+  const float2* src;
+  float2* dst;
+  long src_outer, src_elem;  // strides: tile row (grid.y) and column position
+  long dst_outer, dst_elem;
+  int nvalid;  // source positions present (column length, or ky)
+  int center;  // circular placement offset (0, or ky / 2)
+};
+
+// fused z pass: s (in place) <- IFFTz( FFTz(s) * op(FFTz(k)) * scale )
+struct ZArgs {
+  FftDesc fft;  // length nz
+  int tx;
+  int nbins;
+  float2* s;
+  long s_row, s_plane;  // strides of y and z in s
+  const float2* k;
+  long k_row, k_plane;  // strides of y and z in k
+  int nvalid;           // k planes present (kz, or nz for a dense operand)
+  int center;           // kz / 2, or 0
+  int conj;             // use conj(FFTz(k)) (adjoint / correlation)
+  float scale;          // 1 / (nx*ny*nz)
+};
+
+void launch_gk_xfwd(const XPassArgs& a, long npairs, hipStream_t st);
+void launch_gk_xinv(const XPassArgs& a, long npairs, hipStream_t st);
+void launch_gk_col(const ColArgs& a, int dir, long nouter, hipStream_t st);
+void launch_gk_zfused(const ZArgs& a, long ny, hipStream_t st);
+void generic_kernels_allow_large_lds();
+
+}  // namespace fcamd

@@ -1,0 +1,63 @@
+// kernels_util.hip -- small stand-alone kernels: the unfused pointwise
+// spectrum product (src/convolution3Dfft.cu:41-62, modulateAndNormalize) and
+// a streaming copy used to calibrate HBM byte counters.
+#include "fc_internal.hpp"
+#include "fft_radix.hpp"
+#include "../../include/fourierconvolution_amd.h"
+
+namespace fcamd {
+
+// dst[i] <- scale * dst[i] * src[i] (conj(src[i]) when conj != 0)
+__global__ void __launch_bounds__(256)
+modulate_and_normalize(float2* __restrict__ dst, const float2* __restrict__ src, long n,
+                       float scale, int conj) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x) {
+    const float2 a = dst[i], b = src[i];
+    dst[i] = cscale(conj ? cmulc(a, b) : cmul(a, b), scale);
+  }
+}
+
+// 16-byte streaming copy
+__global__ void __launch_bounds__(256)
+stream_copy(float4* __restrict__ dst, const float4* __restrict__ src, long n) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+}  // namespace fcamd
+
+using namespace fcamd;
+
+extern "C" {
+
+// Pointwise complex product on device arrays of n complex values.
+FourierConvolutionCUDALib_EXPORT int fcamd_modulate_device(void* d_dst, const void* d_src,
+                                                          long long n, float scale, int conj,
+                                                          void* stream) {
+  if (!d_dst || !d_src || n < 0) return kInvalidArgument;
+  if (n == 0) return kOk;
+  long blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(modulate_and_normalize, dim3((unsigned)blocks), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), static_cast<float2*>(d_dst),
+                     static_cast<const float2*>(d_src), (long)n, scale, conj);
+  return hipGetLastError() == hipSuccess ? kOk : kHipError;
+}
+
+// Copies `bytes` (multiple of 16) between device buffers with a plain kernel.
+FourierConvolutionCUDALib_EXPORT int fcamd_stream_copy_device(void* d_dst, const void* d_src,
+                                                             long long bytes, void* stream) {
+  if (!d_dst || !d_src || bytes < 0 || (bytes % 16) != 0) return kInvalidArgument;
+  const long n = (long)(bytes / 16);
+  if (n == 0) return kOk;
+  long blocks = (n + 255) / 256;
+  if (blocks > 1 << 20) blocks = 1 << 20;
+  hipLaunchKernelGGL(stream_copy, dim3((unsigned)blocks), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), static_cast<float4*>(d_dst),
+                     static_cast<const float4*>(d_src), n);
+  return hipGetLastError() == hipSuccess ? kOk : kHipError;
+}
+
+}  // extern "C"

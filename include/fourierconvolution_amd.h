@@ -1,0 +1,55 @@
+/* fourierconvolution_amd.h -- additive C ABI of libFourierConvolutionCUDALib
+ * (MI355X build).  None of these symbols exists in the reference; each one
+ * names the reference behaviour it extends.  Dims use the {x, y, z} order of
+ * convolution3DfftCUDAInPlace (x contiguous).
+ *
+ * Return codes: 0 success, 1 invalid argument, 2 unsupported size,
+ * 3 out of device memory, 4 HIP runtime error, 5 no device, 6 internal.
+ */
+#ifndef FOURIERCONVOLUTION_AMD_H
+#define FOURIERCONVOLUTION_AMD_H
+
+#include "FourierConvolutionCUDALib_Export.h"
+
+#ifdef __cplusplus
+#define FCAMD_API extern "C" FourierConvolutionCUDALib_EXPORT
+#else
+#define FCAMD_API extern FourierConvolutionCUDALib_EXPORT
+#endif
+
+/* Operation selector for the convolution entry points. */
+#define FCAMD_CONVOLVE 0  /* reference semantics (src/convolution3Dfft.cu:41-62) */
+#define FCAMD_CORRELATE 1 /* adjoint w.r.t. the image: conjugated kernel spectrum */
+
+/* Error state of the calling thread (the reference prints and exits instead,
+ * src/book.h:22-30). */
+FCAMD_API int fcamd_last_error(void);
+FCAMD_API const char* fcamd_last_error_message(void);
+
+/* delete[] for the buffers convolution3DfftCUDA / _test return, for callers
+ * that cannot call C++ delete[] (src/convolution3Dfft.cu:240,327). */
+FCAMD_API void fcamd_free_result(float* p);
+
+/* convolution3DfftCUDAInPlace with a return code and an operation selector. */
+FCAMD_API int fcamd_convolve3d_host(float* im, const int* imDim, const float* kernel,
+                                    const int* kernelDim, int op, int devCUDA);
+
+/* Device-resident form of the hot path: every pointer is device memory on
+ * the current device; d_out may alias d_image; `stream` is a hipStream_t
+ * (NULL = the null stream).  d_workspace may be NULL (a per-device pooled
+ * buffer is used; calls on one device are then serialised) or must hold
+ * fcamd_workspace_bytes() bytes.  Asynchronous with respect to the host. */
+FCAMD_API long long fcamd_workspace_bytes(const int* imDim, const int* kernelDim);
+FCAMD_API int fcamd_convolve3d_device(float* d_out, const float* d_image, const int* imDim,
+                                      const float* d_kernel, const int* kernelDim, int op,
+                                      void* d_workspace, long long workspace_bytes, void* stream);
+
+/* FFT radix plan the library uses for a transform length (for diagnostics):
+ * writes up to `cap` radices into `radices`, returns the stage count. */
+FCAMD_API int fcamd_radix_plan(int length, int* radices, int cap);
+
+/* 1 when a compile-time (register-resident) kernel serves this axis length
+ * for the given pass (0 = x forward, 1 = x inverse, 2 = y, 3 = fused z). */
+FCAMD_API int fcamd_has_static_kernel(int pass, int length);
+
+#endif /* FOURIERCONVOLUTION_AMD_H */

@@ -1,0 +1,240 @@
+"""CPU oracle for the 3D FFT convolution hot path (ctypes over liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, ``__graft_entry__.smoke()`` and
+the ``cpu_baseline`` leg of ``bench.py``, always as the checker and never as
+the thing measured or shipped.  The product path never imports this module.
+
+Arrays are numpy C-order ``[nz, ny, nx]`` volumes (x fastest), matching the
+reference's memory layout (``src/convolution3Dfft.cu:394-396``).
+Citations are relative to the reference repository root.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+_REF = None
+
+_i3 = ctypes.POINTER(ctypes.c_int)
+_fp = ctypes.POINTER(ctypes.c_float)
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+def build() -> None:
+    """Compile liboracle.so (and _ref/ when the reference tree is present)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def _lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        lib = ctypes.CDLL(path)
+        lib.fco_convolve_host.argtypes = [_fp, _i3, _fp, _i3, _fp, _i3]
+        lib.fco_fftshift_pad.argtypes = [_fp, _i3, _dp, _i3]
+        lib.fco_fftshift_pad.restype = ctypes.c_int
+        lib.fco_circular_conv_direct.argtypes = [_fp, _i3, _fp, _i3, _dp]
+        lib.fco_circular_conv_direct.restype = ctypes.c_int
+        lib.fco_fft1d.argtypes = [_dp, ctypes.c_int, ctypes.c_long, ctypes.c_int]
+        lib.fco_fft3d.argtypes = [_dp, _i3, ctypes.c_int]
+        lib.fco_fft_conv.argtypes = [_fp, _i3, _fp, _i3, _dp, ctypes.c_int]
+        lib.fco_fft_conv.restype = ctypes.c_int
+        lib.fco_conv_grad_kernel.argtypes = [_dp, _fp, _i3, _i3, _dp]
+        lib.fco_conv_grad_kernel.restype = ctypes.c_int
+        lib.fco_zero_padd_extents.argtypes = [_i3, _i3, ctypes.c_int, _i3, _i3]
+        lib.fco_insert_at_offsets.argtypes = [_fp, _i3, _fp, _i3, _i3]
+        lib.fco_l2norm_ref.argtypes = [_fp, _fp, ctypes.c_long]
+        lib.fco_l2norm_ref.restype = ctypes.c_double
+        lib.fco_rl_multiview.argtypes = [_fp, ctypes.c_int, _i3, _fp, _i3,
+                                         ctypes.c_int, ctypes.c_double, _dp]
+        lib.fco_rl_multiview.restype = ctypes.c_int
+        _LIB = lib
+    return _LIB
+
+
+def ref_available() -> bool:
+    return os.path.exists(os.path.join(_HERE, "_ref", "libref_convolve.so"))
+
+
+def _ref():
+    global _REF
+    if _REF is None:
+        lib = ctypes.CDLL(os.path.join(_HERE, "_ref", "libref_convolve.so"))
+        lib.ref_convolve.argtypes = [_fp, _i3, _fp, _i3, _fp, _i3]
+        _REF = lib
+    return _REF
+
+
+def _ints(v):
+    return (ctypes.c_int * len(v))(*[int(x) for x in v])
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(t)
+
+
+# --------------------------------------------------------------------------
+# restatements
+# --------------------------------------------------------------------------
+def convolve_host(image, kernel, offsets, result=None):
+    """tests/test_algorithms.hpp:9-58 restated in C (bit-identical)."""
+    image = _f32(image)
+    kernel = _f32(kernel)
+    out = np.zeros_like(image) if result is None else result
+    _lib().fco_convolve_host(_ptr(image, _fp), _ints(image.shape),
+                             _ptr(kernel, _fp), _ints(kernel.shape),
+                             _ptr(out, _fp), _ints(offsets))
+    return out
+
+
+def convolve_host_reference(image, kernel, offsets, result=None):
+    """The reference's own template, compiled from its source into _ref/."""
+    image = _f32(image)
+    kernel = _f32(kernel)
+    out = np.zeros_like(image) if result is None else result
+    _ref().ref_convolve(_ptr(image, _fp), _ints(image.shape),
+                        _ptr(kernel, _fp), _ints(kernel.shape),
+                        _ptr(out, _fp), _ints(offsets))
+    return out
+
+
+def fftshift_pad(kernel, image_shape):
+    """src/convolution3Dfft.cu:128-166 with x-fastest axis roles (FP64 out)."""
+    kernel = _f32(kernel)
+    out = np.empty(tuple(image_shape), dtype=np.float64)
+    rc = _lib().fco_fftshift_pad(_ptr(kernel, _fp), _ints(kernel.shape),
+                                 _ptr(out, _dp), _ints(image_shape))
+    if rc:
+        raise ValueError("kernel larger than image")
+    return out
+
+
+def circular_conv_direct(image, kernel):
+    """FP64 direct evaluation of convolution3DfftCUDAInPlace's result."""
+    image = _f32(image)
+    kernel = _f32(kernel)
+    out = np.empty(image.shape, dtype=np.float64)
+    rc = _lib().fco_circular_conv_direct(_ptr(image, _fp), _ints(image.shape),
+                                         _ptr(kernel, _fp), _ints(kernel.shape),
+                                         _ptr(out, _dp))
+    if rc:
+        raise ValueError("kernel larger than image")
+    return out
+
+
+def fft1d(x, sign=-1):
+    """FP64 unnormalised DFT of a 1D complex vector (any length)."""
+    a = np.ascontiguousarray(x, dtype=np.complex128).copy()
+    _lib().fco_fft1d(a.view(np.float64).ctypes.data_as(_dp), a.size, 1, sign)
+    return a
+
+
+def fft3d(x, sign=-1):
+    a = np.ascontiguousarray(x, dtype=np.complex128).copy()
+    _lib().fco_fft3d(a.view(np.float64).ctypes.data_as(_dp), _ints(a.shape), sign)
+    return a
+
+
+def fft_conv(image, kernel, conj_kernel=False):
+    """FP64 FFT circular convolution (own C FFT); conj_kernel -> adjoint."""
+    image = _f32(image)
+    kernel = _f32(kernel)
+    out = np.empty(image.shape, dtype=np.float64)
+    rc = _lib().fco_fft_conv(_ptr(image, _fp), _ints(image.shape),
+                             _ptr(kernel, _fp), _ints(kernel.shape),
+                             _ptr(out, _dp), int(bool(conj_kernel)))
+    if rc:
+        raise ValueError("kernel larger than image")
+    return out
+
+
+def conv_grad_kernel(g, image, kernel_shape):
+    """FP64 direct d(sum g*conv(im,k))/dk on the kernel support."""
+    g = np.ascontiguousarray(g, dtype=np.float64)
+    image = _f32(image)
+    dk = np.empty(tuple(kernel_shape), dtype=np.float64)
+    rc = _lib().fco_conv_grad_kernel(_ptr(g, _dp), _ptr(image, _fp),
+                                     _ints(image.shape), _ints(kernel_shape),
+                                     _ptr(dk, _dp))
+    if rc:
+        raise ValueError("kernel larger than image")
+    return dk
+
+
+def zero_padd(image, kernel_shape, factor=1):
+    """tests/padd_utils.h:100-115,157-171: returns (padded, offsets)."""
+    image = _f32(image)
+    ext = (ctypes.c_int * 3)()
+    off = (ctypes.c_int * 3)()
+    _lib().fco_zero_padd_extents(_ints(image.shape), _ints(kernel_shape),
+                                 int(factor), ext, off)
+    padded = np.empty(tuple(ext), dtype=np.float32)
+    _lib().fco_insert_at_offsets(_ptr(image, _fp), _ints(image.shape),
+                                 _ptr(padded, _fp), ext, off)
+    return padded, tuple(off)
+
+
+def l2norm_ref(a, b):
+    """tests/test_utils.hpp:76-88: sqrt(sum d^2)/N (the reference's metric)."""
+    a = _f32(a)
+    b = _f32(b)
+    return _lib().fco_l2norm_ref(_ptr(a, _fp), _ptr(b, _fp), a.size)
+
+
+def rl_multiview(views, kernels, iterations, eps=1e-6):
+    """FP64 multiview Richardson-Lucy oracle (psi0 = mean of views)."""
+    views = _f32(views)
+    V = views.shape[0]
+    shape = views.shape[1:]
+    kshapes = []
+    flat = []
+    for k in kernels:
+        k = _f32(k)
+        kshapes += list(k.shape)
+        flat.append(k.ravel())
+    kflat = _f32(np.concatenate(flat))
+    psi = np.empty(shape, dtype=np.float64)
+    rc = _lib().fco_rl_multiview(_ptr(views, _fp), V, _ints(shape),
+                                 _ptr(kflat, _fp), _ints(kshapes),
+                                 int(iterations), float(eps), _ptr(psi, _dp))
+    if rc:
+        raise ValueError("kernel larger than image")
+    return psi
+
+
+# --------------------------------------------------------------------------
+# metrics used by every parity test
+# --------------------------------------------------------------------------
+def rel_l2(a, ref):
+    a = np.asarray(a, dtype=np.float64)
+    ref = np.asarray(ref, dtype=np.float64)
+    den = np.linalg.norm(ref.ravel())
+    num = np.linalg.norm((a - ref).ravel())
+    return float(num / den) if den > 0 else float(num)
+
+
+def max_abs(a, ref):
+    return float(np.max(np.abs(np.asarray(a, np.float64) - np.asarray(ref, np.float64))))
+
+
+def numpy_fft_conv(image, kernel, conj_kernel=False):
+    """FP64 FFT circular convolution via numpy's pocketfft (for shapes where
+    the C oracle's O(n log n) recursion is too slow); pinned against
+    fft_conv() on small shapes by the oracle tests."""
+    image = np.asarray(image, dtype=np.float64)
+    pk = fftshift_pad(kernel, image.shape)
+    K = np.fft.rfftn(pk)
+    if conj_kernel:
+        K = np.conj(K)
+    return np.fft.irfftn(np.fft.rfftn(image) * K, s=image.shape)

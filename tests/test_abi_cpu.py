@@ -1,0 +1,116 @@
+"""C-ABI checks that need no GPU: the library loads, exports every symbol the
+public headers declare, and its host-only logic (plans, workspace sizes,
+argument validation, device queries without a device) behaves."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+HEADERS = [os.path.join(ROOT, "include", h)
+           for h in ("convolution3Dfft.h", "fourierconvolution_amd.h")]
+
+
+def _declared_symbols():
+    names = []
+    for h in HEADERS:
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"(?:FUNCTION_PREFIX|FCAMD_API)\s+[\w\s\*]+?\b(\w+)\s*\(", text):
+            names.append(m.group(1))
+    return names
+
+
+def test_headers_declare_the_reference_abi():
+    names = set(_declared_symbols())
+    reference = {  # src/convolution3Dfft.h:29-84
+        "convolution3DfftCUDA_test", "convolution3DfftCUDA", "convolution3DfftCUDAInPlace",
+        "selectDeviceWithHighestComputeCapability", "getCUDAcomputeCapabilityMinorVersion",
+        "getCUDAcomputeCapabilityMajorVersion", "getNumDevicesCUDA", "getNameDeviceCUDA",
+        "getMemDeviceCUDA", "cuda_version", "gpu_mem_needed_mb"}
+    assert reference <= names
+
+
+def test_library_exports_every_declared_symbol(fc):
+    handle = ctypes.CDLL(fc._lib.LIB_PATH)
+    missing = [n for n in _declared_symbols() if not hasattr(handle, n)]
+    assert not missing, missing
+    # and the Python binding knows every one of them
+    assert set(_declared_symbols()) <= set(fc._lib.SIGNATURES)
+
+
+def test_reference_signatures_in_header():
+    text = open(HEADERS[0]).read()
+    assert re.search(r"void\s+convolution3DfftCUDAInPlace\(imageType\* im, int\* imDim, "
+                     r"imageType\* kernel,\s+int\* kernelDim, int devCUDA\)", text)
+    assert "typedef float imageType" in text
+
+
+@pytest.mark.parametrize("n,expect", [(512, [8, 8, 8]), (256, [8, 8, 4]), (600, [8, 3, 5, 5]),
+                                      (105, [3, 5, 7]), (1, []), (173, [173]), (346, [2, 173]),
+                                      (218, [2, 109])])
+def test_radix_plans(fc, n, expect):
+    plan = fc.radix_plan(n)
+    assert plan == expect
+    assert int(np.prod(plan)) == n
+
+
+def test_workspace_bytes(fc):
+    # [nz][ny][nxp] + [kz][ky][nxp] + [kz][ny][nxp] float2, nxp = 257 -> 272,
+    # each region rounded up to 256 bytes
+    def a256(b):
+        return (b + 255) // 256 * 256
+    n = fc.workspace_bytes((256, 512, 512), (15, 15, 15))
+    assert n == a256(8 * 272 * 512 * 256) + a256(8 * 272 * 15 * 15) + a256(8 * 272 * 15 * 512)
+    with pytest.raises(fc.FCError):
+        fc.workspace_bytes((8, 8, 8), (9, 3, 3))
+
+
+def test_gpu_mem_needed_mb(fc):
+    assert fc.gpu_mem_needed_mb([256, 512, 512]) == (256 * 512 * 272 * 8) >> 20
+    assert fc.lib().gpu_mem_needed_mb(None, 3) == -1
+    assert fc.gpu_mem_needed_mb([0, 4, 4]) == -1
+
+
+def test_cuda_version_is_hip_version(fc):
+    v = fc.cuda_version()
+    assert v >= 60000000  # HIP_VERSION of ROCm >= 6
+
+
+def test_errors_do_not_exit_and_leave_input_untouched(fc):
+    im = np.arange(4 * 5 * 6, dtype=np.float32)
+    before = im.copy()
+    k = np.ones(3 * 3 * 9, dtype=np.float32)
+    # kernel longer than the image along x: rejected before any device work
+    fc.lib().convolution3DfftCUDAInPlace(im.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                         fc._lib.ints([6, 5, 4]),
+                                         k.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                         fc._lib.ints([9, 3, 3]), 0)
+    assert fc.lib().fcamd_last_error() == 1
+    assert b"exceeds" in fc.lib().fcamd_last_error_message()
+    assert np.array_equal(im, before)
+    with pytest.raises(fc.FCError):
+        fc.convolution3DfftCUDAInPlace(im, [6, 5, 4], k, [9, 3, 3], 0)
+    assert not fc.lib().convolution3DfftCUDA(None, None, None, None, 0)
+    assert fc.lib().fcamd_last_error() == 1
+
+
+def test_device_queries_without_device(fc):
+    import torch
+    if torch.cuda.device_count() > 0:
+        pytest.skip("a GPU is present")
+    assert fc.getNumDevicesCUDA() == 0
+    assert fc.selectDeviceWithHighestComputeCapability() == -1
+    assert fc.getCUDAcomputeCapabilityMajorVersion(0) == -1
+    assert fc.getMemDeviceCUDA(0) == -1
+    assert fc.getNameDeviceCUDA(0) == ""
+    im = np.ones(8 * 8 * 8, np.float32)
+    k = np.ones(27, np.float32)
+    assert fc.lib().fcamd_convolve3d_host(im.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                          fc._lib.ints([8, 8, 8]),
+                                          k.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                          fc._lib.ints([3, 3, 3]), 0, 0) == 5
+    assert np.all(im == 1)
