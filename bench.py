@@ -1,0 +1,228 @@
+#!/usr/bin/env python3
+"""Benchmark of the 3D FFT convolution hot path on one MI355X.
+
+Workload: plain in-place-semantics circular convolution of a 512x512x256
+(x, y, z) float32 image (seeded uniform [0,1)) with a 15x15x15 anisotropic
+Gaussian PSF (sigma 1.5, 1.5, 3.0), inputs resident in HBM.  One step = one
+full convolution (kernel spectrum included), through the library's C ABI
+(fcamd_convolve3d_device) on a torch-allocated buffer and stream.
+
+Prints ONE JSON line (rank 0): voxels/s, the whole-convolution HBM roofline
+fraction (bytes_alg = 8N + 64Nc + 4K per convolution), the per-pass
+roofline of the dominant kernel from HIP events recorded around each pass
+over a second timed region, and the CPU baseline: the reference's host
+convolution (tests/test_algorithms.hpp:9-58, compiled into oracle/_ref) on a
+bounded slab of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import threading
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+
+
+def pass_bytes(nx, ny, nz, kx, ky, kz):
+    """Algorithmic HBM bytes per pass launch (the kernel transforms act on
+    its compact support and count only their reads/writes of it)."""
+    N = nx * ny * nz
+    nb = nx // 2 + 1
+    Nc = nb * ny * nz
+    return {
+        "kernel_rows_xfwd": 4 * kx * ky * kz + 8 * nb * ky * kz,
+        "kernel_planes_yfwd": 8 * nb * ky * kz + 8 * nb * ny * kz,
+        "image_xfwd": 4 * N + 8 * Nc,
+        "image_yfwd": 16 * Nc,
+        "zfused": 16 * Nc,
+        "image_yinv": 16 * Nc,
+        "image_xinv": 8 * Nc + 4 * N,
+    }, 8 * N + 64 * Nc + 4 * kx * ky * kz
+
+
+def cpu_baseline(img, psf, planes_per_thread=2):
+    """Reference host convolution (oracle/_ref, else its C restatement) on a
+    slab of output z-planes, one slab per host thread."""
+    import numpy as np
+    from oracle import oracle as O
+    kind = "reference" if O.ref_available() else "port"
+    fn = O.convolve_host_reference if kind == "reference" else O.convolve_host
+    threads = max(1, min(16, os.cpu_count() or 1))
+    kz, ky, kx = psf.shape
+    hz, hy, hx = kz // 2, ky // 2, kx // 2
+    nz = img.shape[0]
+    padded = np.pad(img, ((hz, hz), (hy, hy), (hx, hx)))
+    slabs = []
+    for t in range(threads):
+        z0 = (t * planes_per_thread) % max(1, nz - planes_per_thread)
+        sub = np.ascontiguousarray(padded[z0:z0 + planes_per_thread + 2 * hz])
+        slabs.append(sub)
+    outs = [None] * threads
+
+    def work(i):
+        outs[i] = fn(slabs[i], psf, (hz, hy, hx))
+
+    fn(np.ascontiguousarray(padded[:2 * hz + 1, :2 * hy + 8, :2 * hx + 8]), psf, (hz, hy, hx))
+    t0 = time.perf_counter()
+    ths = [threading.Thread(target=work, args=(i,)) for i in range(threads)]
+    for t in ths:
+        t.start()
+    for t in ths:
+        t.join()
+    dt = time.perf_counter() - t0
+    voxels = threads * planes_per_thread * img.shape[1] * img.shape[2]
+    return {
+        "value": voxels / dt,
+        "unit": "voxels/s",
+        "cores": threads,
+        "kind": kind,
+        "sample": (f"{threads * planes_per_thread} of {nz} output z-planes of the "
+                   f"{img.shape[2]}x{img.shape[1]}x{nz} k{kx}x{ky}x{kz} workload "
+                   f"(zero-padded direct convolution, tests/test_algorithms.hpp:9-58), "
+                   f"{planes_per_thread} planes per thread, {dt:.1f} s"),
+    }
+
+
+def load_traffic(workload):
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(the path's sharding is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import fourierconvolutioncudalib_amd as fc
+    from fourierconvolutioncudalib_amd import synth
+
+    cfg = synth.METRIC
+    img_np, psf_np = synth.workload(cfg)
+    nz, ny, nx = img_np.shape
+    kz, ky, kx = psf_np.shape
+    dev = torch.device("cuda:0")
+    img = torch.from_numpy(img_np).to(dev)
+    psf = torch.from_numpy(psf_np).to(dev)
+    out = torch.empty_like(img)
+    ws = torch.empty(fc.workspace_bytes(img.shape, psf.shape), dtype=torch.uint8, device=dev)
+    stream = torch.cuda.current_stream(dev)
+    lib = fc.lib()
+
+    def step():
+        fc.convolve3d(img, psf, out=out, workspace=ws, stream=stream)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+
+    # timed region 1: plain steps
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(args.steps):
+        step()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_step = e0.elapsed_time(e1) / args.steps
+
+    # timed region 2: the same steps with HIP events around every pass
+    lib.fcamd_pass_timing(1)
+    torch.cuda.synchronize()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    lib.fcamd_pass_timing(0)
+    ms = (ctypes.c_double * 16)()
+    cnt = (ctypes.c_int * 16)()
+    npass = lib.fcamd_pass_times(ms, cnt, 16)
+    per_pass = {}
+    for i in range(npass):
+        if cnt[i]:
+            per_pass[lib.fcamd_pass_name(i).decode()] = ms[i] / cnt[i]
+
+    pbytes, conv_bytes = pass_bytes(nx, ny, nz, kx, ky, kz)
+    dominant = max(per_pass, key=per_pass.get)
+    t_dom = per_pass[dominant] * 1e-3
+    achieved = pbytes[dominant] / t_dom
+    traffic = load_traffic(cfg["name"])
+    N = nx * ny * nz
+
+    # correctness spot check against the FP64 spectrum product is in the tests;
+    # here only a finiteness guard on the produced volume
+    assert torch.isfinite(out).all().item()
+
+    # drop-in host-pointer call (PCIe-inclusive), reported beside the value
+    im_host = img_np.copy()
+    t_host = []
+    for _ in range(3):
+        im_host[:] = img_np
+        t1 = time.perf_counter()
+        fc.convolution3DfftCUDAInPlace(im_host.reshape(-1), [nx, ny, nz], psf_np.reshape(-1),
+                                       [kx, ky, kz], 0)
+        t_host.append(time.perf_counter() - t1)
+
+    result = {
+        "metric": "conv voxels/s + %HBM roofline at 512^2x256 fp32",
+        "value": N / (ms_step * 1e-3),
+        "unit": "voxels/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (SplitMix64 uniform [0,1) image, Gaussian PSF)",
+        "config": {"workload": cfg["name"], "image_xyz": [nx, ny, nz], "kernel_xyz": [kx, ky, kz],
+                   "op": "convolve", "parallelism": "single-gpu"},
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dominant,
+            "achieved": achieved / 1e9,
+            "peak": HBM_PEAK / 1e9,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK,
+            "traffic": traffic.get(dominant) if traffic else None,
+        },
+        "conv_roofline": {
+            "bytes_alg": conv_bytes,
+            "achieved_GBps": conv_bytes / (ms_step * 1e-3) / 1e9,
+            "frac": conv_bytes / (ms_step * 1e-3) / HBM_PEAK,
+        },
+        "pass_ms": {k: round(v, 5) for k, v in per_pass.items()},
+        "pass_GBps": {k: round(pbytes[k] / (v * 1e-3) / 1e9, 1) for k, v in per_pass.items()},
+        "wall_s_timed_region": wall,
+        "host_api_ms_pcie_inclusive": min(t_host) * 1e3,
+    }
+    if not args.no_cpu_baseline:
+        result["cpu_baseline"] = cpu_baseline(img_np, psf_np)
+    else:
+        result["cpu_baseline"] = None
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

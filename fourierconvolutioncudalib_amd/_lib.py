@@ -44,6 +44,9 @@ SIGNATURES = {
     "fcamd_convolve3d_device": (_i, [_vp, _vp, _ip, _vp, _ip, _i, _vp, _ll, _vp]),
     "fcamd_radix_plan": (_i, [_i, _ip, _i]),
     "fcamd_has_static_kernel": (_i, [_i, _i]),
+    "fcamd_pass_timing": (None, [_i]),
+    "fcamd_pass_times": (_i, [ctypes.POINTER(ctypes.c_double), _ip, _i]),
+    "fcamd_pass_name": (ctypes.c_char_p, [_i]),
     "fcamd_modulate_device": (_i, [_vp, _vp, _ll, ctypes.c_float, _i, _vp]),
     "fcamd_stream_copy_device": (_i, [_vp, _vp, _ll, _vp]),
 }
@@ -65,6 +68,14 @@ def lib():
             raise ImportError(
                 f"{LIB_PATH} is missing: build it with "
                 f"`make -C {CSRC}` (or __graft_entry__.build())")
+        # torch ships its own libamdhip64.so.7 (same soname as the system
+        # one).  Two HIP/HSA runtimes cannot share a process, and the first
+        # one loaded wins the soname, so load torch's before ours when torch
+        # is installed; the library then runs on the runtime torch uses.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         handle = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(handle, name)

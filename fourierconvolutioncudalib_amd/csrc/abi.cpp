@@ -220,8 +220,11 @@ void getNameDeviceCUDA(int devCUDA, char* name) {
   char buf[256];
   std::memset(buf, 0, sizeof(buf));
   hipDeviceProp_t p;
-  if (hipGetDeviceProperties(&p, devCUDA) == hipSuccess)
-    std::strncpy(buf, p.name, sizeof(buf) - 1);
+  if (hipGetDeviceProperties(&p, devCUDA) == hipSuccess) {
+    // the marketing name can be empty (no name table on the host); the
+    // architecture name (e.g. gfx950:sramecc+:xnack-) is always set
+    std::strncpy(buf, p.name[0] ? p.name : p.gcnArchName, sizeof(buf) - 1);
+  }
   std::memcpy(name, buf, sizeof(buf));
 }
 
@@ -276,6 +279,22 @@ int fcamd_convolve3d_device(float* d_out, const float* d_image, const int* imDim
     }
   });
 }
+
+void fcamd_pass_timing(int enable) { pass_timing_enable(enable != 0); }
+
+int fcamd_pass_times(double* ms, int* counts, int cap) {
+  double m[kNumPasses];
+  int c[kNumPasses];
+  int rc = guarded([&] { pass_timing_collect(m, c); });
+  if (rc != kOk) return -rc;
+  for (int i = 0; i < kNumPasses && i < cap; ++i) {
+    ms[i] = m[i];
+    counts[i] = c[i];
+  }
+  return kNumPasses;
+}
+
+const char* fcamd_pass_name(int pass) { return pass_name(pass); }
 
 int fcamd_radix_plan(int length, int* radices, int cap) {
   if (length < 1) return -1;

@@ -139,6 +139,76 @@ WorkspaceLayout workspace_layout(const ConvShape& s) {
   return w;
 }
 
+// ---------------------------------------------------------------------------
+// per-pass event timing
+// ---------------------------------------------------------------------------
+namespace {
+std::mutex g_pt_mu;
+bool g_pt_on = false;
+struct PassEvents {
+  int pass;
+  hipEvent_t a, b;
+};
+std::vector<PassEvents> g_pt_events;
+
+class PassScope {
+ public:
+  PassScope(int pass, hipStream_t st) : st_(st) {
+    std::lock_guard<std::mutex> lk(g_pt_mu);
+    if (!g_pt_on) return;
+    rec_.pass = pass;
+    FC_HIP_CHECK(hipEventCreate(&rec_.a));
+    FC_HIP_CHECK(hipEventCreate(&rec_.b));
+    FC_HIP_CHECK(hipEventRecord(rec_.a, st_));
+    active_ = true;
+  }
+  ~PassScope() {
+    if (!active_) return;
+    (void)hipEventRecord(rec_.b, st_);
+    std::lock_guard<std::mutex> lk(g_pt_mu);
+    g_pt_events.push_back(rec_);
+  }
+
+ private:
+  hipStream_t st_;
+  PassEvents rec_{};
+  bool active_ = false;
+};
+}  // namespace
+
+const char* pass_name(int p) {
+  static const char* names[kNumPasses] = {"kernel_rows_xfwd", "kernel_planes_yfwd", "image_xfwd",
+                                          "image_yfwd",       "zfused",             "image_yinv",
+                                          "image_xinv"};
+  return (p >= 0 && p < kNumPasses) ? names[p] : "?";
+}
+
+void pass_timing_enable(bool on) {
+  std::lock_guard<std::mutex> lk(g_pt_mu);
+  g_pt_on = on;
+}
+
+void pass_timing_collect(double* ms, int* count) {
+  std::vector<PassEvents> ev;
+  {
+    std::lock_guard<std::mutex> lk(g_pt_mu);
+    ev.swap(g_pt_events);
+  }
+  for (int p = 0; p < kNumPasses; ++p) {
+    ms[p] = 0.0;
+    count[p] = 0;
+  }
+  for (auto& e : ev) {
+    float t = 0.f;
+    FC_HIP_CHECK(hipEventSynchronize(e.b));
+    FC_HIP_CHECK(hipEventElapsedTime(&t, e.a, e.b));
+    ms[e.pass] += t;
+    count[e.pass] += 1;
+    (void)hipEventDestroy(e.a);
+    (void)hipEventDestroy(e.b);
+  }
+}
+
 void enqueue_convolution(const ConvShape& s, float* d_out, const float* d_image,
                          const float* d_kernel, Op op, void* ws, hipStream_t st) {
   std::call_once(g_attr_once, [] {
@@ -156,6 +226,7 @@ void enqueue_convolution(const ConvShape& s, float* d_out, const float* d_image,
 
   // K1: kernel rows
   {
+    PassScope ps(kPassKernelRows, st);
     XPassArgs a{};
     a.fft = fx;
     a.nrows = (long)s.ky * s.kz;
@@ -172,6 +243,7 @@ void enqueue_convolution(const ConvShape& s, float* d_out, const float* d_image,
   }
   // K2: kernel planes
   {
+    PassScope ps(kPassKernelPlanes, st);
     ColArgs a{};
     a.fft = fy;
     a.tx = col_tile(s.ny, 2);
@@ -188,7 +260,9 @@ void enqueue_convolution(const ConvShape& s, float* d_out, const float* d_image,
   }
   // P1: image rows
   const long nrows = (long)s.ny * s.nz;
-  if (!static_xfwd(s, fx, d_image, S, st)) {
+  {
+    PassScope ps(kPassXForward, st);
+    if (!static_xfwd(s, fx, d_image, S, st)) {
     XPassArgs a{};
     a.fft = fx;
     a.nrows = nrows;
@@ -199,6 +273,7 @@ void enqueue_convolution(const ConvShape& s, float* d_out, const float* d_image,
     a.src_row = s.nx;
     a.dst_row = nxp;
     launch_gk_xfwd(a, (nrows + 1) / 2, st);
+    }
   }
   // P2 / P4: y passes in place
   ColArgs ya{};
@@ -211,9 +286,13 @@ void enqueue_convolution(const ConvShape& s, float* d_out, const float* d_image,
   ya.src_elem = ya.dst_elem = nxp;
   ya.nvalid = s.ny;
   ya.center = 0;
-  if (s.ny > 1 && !static_ycol(s, fy, S, -1, st)) launch_gk_col(ya, -1, s.nz, st);
+  if (s.ny > 1) {
+    PassScope ps(kPassYForward, st);
+    if (!static_ycol(s, fy, S, -1, st)) launch_gk_col(ya, -1, s.nz, st);
+  }
   // P3: fused z pass
   {
+    PassScope ps(kPassZFused, st);
     ZArgs a{};
     a.fft = fz;
     a.tx = col_tile(s.nz, 3);
@@ -230,9 +309,14 @@ void enqueue_convolution(const ConvShape& s, float* d_out, const float* d_image,
     a.scale = (float)(1.0 / (double)s.voxels());
     if (!static_zfused(s, a, st)) launch_gk_zfused(a, s.ny, st);
   }
-  if (s.ny > 1 && !static_ycol(s, fy, S, +1, st)) launch_gk_col(ya, +1, s.nz, st);
+  if (s.ny > 1) {
+    PassScope ps(kPassYInverse, st);
+    if (!static_ycol(s, fy, S, +1, st)) launch_gk_col(ya, +1, s.nz, st);
+  }
   // P5: back to real rows
-  if (!static_xinv(s, fx, S, d_out, st)) {
+  {
+    PassScope ps(kPassXInverse, st);
+    if (!static_xinv(s, fx, S, d_out, st)) {
     XPassArgs a{};
     a.fft = fx;
     a.nrows = nrows;
@@ -243,6 +327,7 @@ void enqueue_convolution(const ConvShape& s, float* d_out, const float* d_image,
     a.src_row = nxp;
     a.dst_row = s.nx;
     launch_gk_xinv(a, (nrows + 1) / 2, st);
+    }
   }
   FC_HIP_CHECK(hipGetLastError());
 }

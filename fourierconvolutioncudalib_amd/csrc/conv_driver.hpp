@@ -42,6 +42,25 @@ std::vector<int> radix_plan(int L);
 
 enum class Op : int { kConvolve = 0, kCorrelate = 1 };
 
+// Passes of one convolution, in launch order (for per-pass event timing).
+enum Pass : int {
+  kPassKernelRows = 0,
+  kPassKernelPlanes,
+  kPassXForward,
+  kPassYForward,
+  kPassZFused,
+  kPassYInverse,
+  kPassXInverse,
+  kNumPasses
+};
+const char* pass_name(int p);
+
+// Optional HIP-event timing of every pass launch (off by default).
+void pass_timing_enable(bool on);
+// Waits for the recorded events and returns the summed milliseconds and the
+// launch count per pass since the last call (then resets).
+void pass_timing_collect(double* ms, int* count);
+
 // Enqueues the full convolution on `stream`: d_out may alias d_image.
 // `ws` must hold workspace_layout(s).total() bytes.
 void enqueue_convolution(const ConvShape& s, float* d_out, const float* d_image,

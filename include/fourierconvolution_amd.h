@@ -44,6 +44,29 @@ FCAMD_API int fcamd_convolve3d_device(float* d_out, const float* d_image, const 
                                       const float* d_kernel, const int* kernelDim, int op,
                                       void* d_workspace, long long workspace_bytes, void* stream);
 
+/* Per-pass HIP-event timing of the convolution passes (diagnostics and the
+ * benchmark's roofline figures).  While enabled every pass launch is
+ * bracketed by two events on its stream.  fcamd_pass_times waits for them,
+ * writes summed milliseconds and launch counts per pass (up to `cap`
+ * passes), resets, and returns the pass count (negative error code on
+ * failure).  Passes: 0 kernel rows x-FFT, 1 kernel planes y-FFT,
+ * 2 image x-R2C, 3 image y-FFT, 4 fused z pass, 5 inverse y-FFT,
+ * 6 x-C2R. */
+FCAMD_API void fcamd_pass_timing(int enable);
+FCAMD_API int fcamd_pass_times(double* ms, int* counts, int cap);
+FCAMD_API const char* fcamd_pass_name(int pass);
+
+/* Unfused pointwise spectrum product on device arrays of n complex values:
+ * dst <- scale * dst * src (conj(src) when conj != 0); the stand-alone form
+ * of src/convolution3Dfft.cu:47-62. */
+FCAMD_API int fcamd_modulate_device(void* d_dst, const void* d_src, long long n, float scale,
+                                    int conj, void* stream);
+
+/* Plain streaming copy of `bytes` (multiple of 16) between device buffers,
+ * used to calibrate HBM byte counters and measure achievable bandwidth. */
+FCAMD_API int fcamd_stream_copy_device(void* d_dst, const void* d_src, long long bytes,
+                                       void* stream);
+
 /* FFT radix plan the library uses for a transform length (for diagnostics):
  * writes up to `cap` radices into `radices`, returns the stage count. */
 FCAMD_API int fcamd_radix_plan(int length, int* radices, int cap);
