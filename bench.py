@@ -173,6 +173,22 @@ def main():
     # here only a finiteness guard on the produced volume
     assert torch.isfinite(out).all().item()
 
+    # achievable HBM bandwidth: plain 16-byte streaming copy, 1 GiB each way
+    nbytes = 1 << 30
+    src = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+    dst = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+    for _ in range(2):
+        fc._lib.check(lib.fcamd_stream_copy_device(dst.data_ptr(), src.data_ptr(), nbytes,
+                                                   ctypes.c_void_p(stream.cuda_stream)))
+    e0.record(stream)
+    for _ in range(10):
+        lib.fcamd_stream_copy_device(dst.data_ptr(), src.data_ptr(), nbytes,
+                                     ctypes.c_void_p(stream.cuda_stream))
+    e1.record(stream)
+    torch.cuda.synchronize()
+    copy_GBps = 2 * nbytes * 10 / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    del src, dst
+
     # drop-in host-pointer call (PCIe-inclusive), reported beside the value
     im_host = img_np.copy()
     t_host = []
@@ -212,6 +228,7 @@ def main():
             "achieved_GBps": conv_bytes / (ms_step * 1e-3) / 1e9,
             "frac": conv_bytes / (ms_step * 1e-3) / HBM_PEAK,
         },
+        "hbm_copy_GBps": copy_GBps,
         "pass_ms": {k: round(v, 5) for k, v in per_pass.items()},
         "pass_GBps": {k: round(pbytes[k] / (v * 1e-3) / 1e9, 1) for k, v in per_pass.items()},
         "wall_s_timed_region": wall,

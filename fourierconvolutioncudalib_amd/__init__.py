@@ -157,6 +157,18 @@ def radix_plan(length):
     return [buf[i] for i in range(n)]
 
 
+def static_lengths():
+    """Transform lengths served by compile-time (register-resident) kernels."""
+    buf = (ctypes.c_int * 256)()
+    n = lib().fcamd_static_lengths(buf, 256)
+    return [buf[i] for i in range(min(n, 256))]
+
+
+def use_static_kernels(enable: bool) -> None:
+    """False routes every pass through the runtime-length kernels."""
+    lib().fcamd_use_static_kernels(1 if enable else 0)
+
+
 def _stream_handle(stream):
     if stream is None:
         import torch

@@ -1,0 +1,181 @@
+// fft_static.hpp -- compile-time-length Stockham FFT held in registers.
+//
+// A transform of length L is served by T = L / Rmax threads (Rmax = largest
+// radix of the plan).  In stage s (radix R, Ns = product of the earlier
+// radices, M = L / R) thread t owns butterflies b = t + i*T (i < NB,
+// predicated b < M); butterfly b reads positions b + r*M and writes
+// positions (b / Ns)*Ns*R + b % Ns + r*Ns (Stockham auto-sort, so the result
+// is in natural order).  Stage 0 inputs come straight from the caller's
+// loader and last-stage outputs stay in registers at positions
+// b + r*(L / R_last); only the exchanges between stages go through LDS.
+//
+// A plan run in reverse order (REV) starts with the radix the forward plan
+// ended with, so the same thread already holds exactly the positions the
+// reversed plan's first stage needs: a forward transform, a pointwise
+// product and an inverse transform chain with no LDS exchange in between.
+#pragma once
+
+#include "fc_internal.hpp"
+#include "fft_radix.hpp"
+
+namespace fcamd {
+
+struct StaticPlanData {
+  int n = 0;
+  int r[16] = {};
+};
+
+// Balanced radix plan for a 7-smooth length: powers of two as 8s with the
+// remainder as one 4 (2^(3k+2)) or two 4s (2^(3k+1)); then 3s, 5s, 7s.
+constexpr StaticPlanData static_plan(int L) {
+  StaticPlanData p{};
+  int a = 0;
+  int n = L;
+  while (n % 2 == 0) { n /= 2; ++a; }
+  int eights = a / 3, rem = a % 3;
+  if (rem == 1 && eights > 0) { --eights; p.r[p.n++] = 4; p.r[p.n++] = 4; }
+  for (int i = 0; i < eights; ++i) p.r[p.n++] = 8;
+  if (rem == 2) p.r[p.n++] = 4;
+  if (rem == 1 && a == 1) p.r[p.n++] = 2;
+  while (n % 3 == 0) { p.r[p.n++] = 3; n /= 3; }
+  while (n % 5 == 0) { p.r[p.n++] = 5; n /= 5; }
+  while (n % 7 == 0) { p.r[p.n++] = 7; n /= 7; }
+  if (n != 1) p.n = -1;  // not 7-smooth: no static plan
+  return p;
+}
+
+constexpr int plan_rmax(const StaticPlanData& p) {
+  int m = 1;
+  for (int i = 0; i < p.n; ++i) m = p.r[i] > m ? p.r[i] : m;
+  return m;
+}
+
+template <int L, bool REV>
+struct PlanOf {
+  static constexpr StaticPlanData data = static_plan(L);
+  static_assert(data.n > 0, "length without a static plan");
+  static constexpr int S = data.n;
+  static constexpr int RMAX = plan_rmax(data);
+  static constexpr int T = L / RMAX;
+  static constexpr int R(int s) { return REV ? data.r[S - 1 - s] : data.r[s]; }
+  static constexpr int Ns(int s) {
+    int v = 1;
+    for (int i = 0; i < s; ++i) v *= R(i);
+    return v;
+  }
+  static constexpr int M(int s) { return L / R(s); }
+  static constexpr int NB(int s) { return (M(s) + T - 1) / T; }
+  static constexpr int E(int s) { return NB(s) * R(s); }
+  static constexpr int MAXE() {
+    int m = 0;
+    for (int s = 0; s < S; ++s) m = E(s) > m ? E(s) : m;
+    return m;
+  }
+};
+
+// One stage's butterflies on the thread's registers (inputs already loaded
+// at positions b + r*M, outputs left in place).
+template <int L, bool REV, int DIR, int s, int MAXE>
+__device__ __forceinline__ void sfft_compute(float2 (&v)[MAXE], int t, const float2* __restrict__ tw) {
+  using P = PlanOf<L, REV>;
+  constexpr int R = P::R(s), Ns = P::Ns(s), M = P::M(s), NB = P::NB(s);
+  constexpr int twstep = L / (Ns * R);
+#pragma unroll
+  for (int i = 0; i < NB; ++i) {
+    const int b = t + i * P::T;
+    if ((M % P::T) != 0 && b >= M) continue;
+    float2* x = &v[i * R];
+    if (Ns > 1) {
+      const int k = b % Ns;
+#pragma unroll
+      for (int r = 1; r < R; ++r) x[r] = cmul(x[r], twiddle<DIR>(tw, r * k * twstep));
+    }
+    Dft<R, DIR>::run(x);
+  }
+}
+
+// Writes stage s outputs to LDS through `addr(pos)`.
+template <int L, bool REV, int s, int MAXE, class Addr>
+__device__ __forceinline__ void sfft_store_lds(const float2 (&v)[MAXE], int t, float2* lds, Addr addr) {
+  using P = PlanOf<L, REV>;
+  constexpr int R = P::R(s), Ns = P::Ns(s), M = P::M(s), NB = P::NB(s);
+#pragma unroll
+  for (int i = 0; i < NB; ++i) {
+    const int b = t + i * P::T;
+    if ((M % P::T) != 0 && b >= M) continue;
+    const int k = b % Ns;
+    const int base = (b - k) * R + k;
+#pragma unroll
+    for (int r = 0; r < R; ++r) lds[addr(base + r * Ns)] = v[i * R + r];
+  }
+}
+
+// Reads stage s inputs from LDS.
+template <int L, bool REV, int s, int MAXE, class Addr>
+__device__ __forceinline__ void sfft_load_lds(float2 (&v)[MAXE], int t, const float2* lds, Addr addr) {
+  using P = PlanOf<L, REV>;
+  constexpr int R = P::R(s), M = P::M(s), NB = P::NB(s);
+#pragma unroll
+  for (int i = 0; i < NB; ++i) {
+    const int b = t + i * P::T;
+    if ((M % P::T) != 0 && b >= M) continue;
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[i * R + r] = lds[addr(b + r * M)];
+  }
+}
+
+// Stage-0 input positions of the thread: calls f(slot, pos) for every valid
+// input (slot indexes the register array).
+template <int L, bool REV, class F>
+__device__ __forceinline__ void sfft_for_inputs(int t, F f) {
+  using P = PlanOf<L, REV>;
+  constexpr int R = P::R(0), M = P::M(0), NB = P::NB(0);
+#pragma unroll
+  for (int i = 0; i < NB; ++i) {
+    const int b = t + i * P::T;
+    if ((M % P::T) != 0 && b >= M) continue;
+#pragma unroll
+    for (int r = 0; r < R; ++r) f(i * R + r, b + r * M);
+  }
+}
+
+// Last-stage output positions of the thread: f(slot, pos).
+template <int L, bool REV, class F>
+__device__ __forceinline__ void sfft_for_outputs(int t, F f) {
+  using P = PlanOf<L, REV>;
+  constexpr int s = P::S - 1;
+  constexpr int R = P::R(s), M = P::M(s), NB = P::NB(s);
+#pragma unroll
+  for (int i = 0; i < NB; ++i) {
+    const int b = t + i * P::T;
+    if ((M % P::T) != 0 && b >= M) continue;
+#pragma unroll
+    for (int r = 0; r < R; ++r) f(i * R + r, b + r * M);
+  }
+}
+
+// Stages 1..S-1 (stage 0 inputs must already be in v); the LDS region is
+// reused, so the caller must not have other threads reading it.
+template <int L, bool REV, int DIR, int s, int MAXE, class Addr>
+__device__ __forceinline__ void sfft_rest(float2 (&v)[MAXE], int t, float2* lds, Addr addr,
+                                          const float2* __restrict__ tw) {
+  using P = PlanOf<L, REV>;
+  if constexpr (s < P::S) {
+    sfft_store_lds<L, REV, s - 1>(v, t, lds, addr);
+    __syncthreads();
+    sfft_load_lds<L, REV, s>(v, t, lds, addr);
+    __syncthreads();
+    sfft_compute<L, REV, DIR, s>(v, t, tw);
+    sfft_rest<L, REV, DIR, s + 1>(v, t, lds, addr, tw);
+  }
+}
+
+// Whole transform: stage 0 compute, then the remaining stages.
+template <int L, bool REV, int DIR, int MAXE, class Addr>
+__device__ __forceinline__ void sfft_run(float2 (&v)[MAXE], int t, float2* lds, Addr addr,
+                                         const float2* __restrict__ tw) {
+  sfft_compute<L, REV, DIR, 0>(v, t, tw);
+  sfft_rest<L, REV, DIR, 1>(v, t, lds, addr, tw);
+}
+
+}  // namespace fcamd

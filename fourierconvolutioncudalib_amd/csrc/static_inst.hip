@@ -1,0 +1,42 @@
+// static_inst.hip -- instantiates the compile-time kernels for one group of
+// lengths (FC_GROUP, set by the Makefile) and registers them at load time.
+// The groups split the instantiations over several objects so they compile
+// in parallel.
+#include "static_kernels_impl.hpp"
+
+#ifndef FC_GROUP
+#error "FC_GROUP must be defined"
+#endif
+
+// Lengths with compile-time kernels: the image extents of the benchmark
+// workloads and the uneven-block menu, plus common microscopy sizes.
+#if FC_GROUP == 0
+#define FC_LENGTHS(X) X(512) X(256)
+#elif FC_GROUP == 1
+#define FC_LENGTHS(X) X(600) X(500) X(240)
+#elif FC_GROUP == 2
+#define FC_LENGTHS(X) X(384) X(192) X(128) X(1024)
+#elif FC_GROUP == 3
+#define FC_LENGTHS(X) X(64) X(80) X(96) X(105) X(112)
+#elif FC_GROUP == 4
+#define FC_LENGTHS(X) X(120) X(144) X(160) X(210) X(224)
+#elif FC_GROUP == 5
+#define FC_LENGTHS(X) X(32) X(48) X(288) X(320) X(400) X(448) X(640) X(768)
+#else
+#define FC_LENGTHS(X)
+#endif
+
+namespace fcamd {
+namespace {
+
+struct Registrar {
+  Registrar() {
+#define FC_REG(L) static_register(L, make_static_entry<L>());
+    FC_LENGTHS(FC_REG)
+#undef FC_REG
+  }
+};
+Registrar g_registrar;
+
+}  // namespace
+}  // namespace fcamd

@@ -75,4 +75,12 @@ FCAMD_API int fcamd_radix_plan(int length, int* radices, int cap);
  * for the given pass (0 = x forward, 1 = x inverse, 2 = y, 3 = fused z). */
 FCAMD_API int fcamd_has_static_kernel(int pass, int length);
 
+/* Lists the lengths with compile-time kernels (up to `cap`); returns how many
+ * there are. */
+FCAMD_API int fcamd_static_lengths(int* lengths, int cap);
+
+/* enable = 0 routes every pass through the runtime-length kernels (for
+ * cross-checks and comparisons); 1 restores the default.  Process-wide. */
+FCAMD_API void fcamd_use_static_kernels(int enable);
+
 #endif /* FOURIERCONVOLUTION_AMD_H */

@@ -1,0 +1,77 @@
+"""Every transform length with a compile-time kernel, on every axis, against
+the FP64 oracle; compile-time and runtime-length kernels against each other
+at the benchmark shape; runtime-length kernels (incl. prime radices) on
+lengths without compile-time kernels."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def _lengths():
+    import fourierconvolutioncudalib_amd as fc
+    return fc.static_lengths()
+
+
+def _check(fc, oracle, shape_zyx, kshape_zyx, seed):
+    from fourierconvolutioncudalib_amd import synth
+    img = synth.uniform(shape_zyx, seed)
+    k = synth.gaussian_psf(kshape_zyx, (1.3, 0.9, 1.7))
+    k[0, 0, 0] += 0.25  # break the PSF's symmetry so a flip would show
+    out = fc.convolve(img, k)
+    ref = oracle.numpy_fft_conv(img, k)
+    err = oracle.rel_l2(out, ref)
+    assert err < TOL, (shape_zyx, kshape_zyx, err)
+    return err
+
+
+@pytest.mark.parametrize("axis", ["x", "y", "z"])
+def test_every_static_length(fc, oracle, axis):
+    lengths = _lengths()
+    assert {512, 256, 600, 500, 240, 384, 192, 128}.issubset(lengths)
+    for i, L in enumerate(lengths):
+        if axis == "x":
+            shape = (6, 10, L)
+        elif axis == "y":
+            shape = (6, L, 24)
+        else:
+            shape = (L, 10, 24)
+        kshape = tuple(min(s, k) for s, k in zip(shape, (5, 7, 3)))
+        _check(fc, oracle, shape, kshape, 100 + i)
+
+
+@pytest.mark.parametrize("shape", [(13, 17, 19), (15, 19, 21), (46, 46, 106), (22, 26, 34),
+                                   (11, 121, 9), (1, 1, 3000), (2, 1100, 1), (700, 3, 2)])
+def test_runtime_length_kernels(fc, oracle, shape):
+    """Lengths without compile-time kernels (generic radices 11, 13, 17, 19,
+    23, 53 and long axes)."""
+    kshape = tuple(min(s, k) for s, k in zip(shape, (3, 5, 7)))
+    _check(fc, oracle, shape, kshape, 7)
+
+
+def test_static_matches_runtime_kernels_at_benchmark_shape(fc, oracle):
+    import torch
+    from fourierconvolutioncudalib_amd import synth
+    img, psf = synth.workload(synth.METRIC)
+    ti = torch.from_numpy(img).cuda()
+    tk = torch.from_numpy(psf).cuda()
+    fast = fc.convolve3d(ti, tk).cpu().numpy()
+    fc.use_static_kernels(False)
+    try:
+        slow = fc.convolve3d(ti, tk).cpu().numpy()
+    finally:
+        fc.use_static_kernels(True)
+    assert oracle.rel_l2(fast, slow) < 2e-6
+    ref = oracle.numpy_fft_conv(img, psf)
+    assert oracle.rel_l2(fast, ref) < TOL
+    assert oracle.rel_l2(slow, ref) < TOL
+
+
+def test_correlate_static(fc, oracle):
+    from fourierconvolutioncudalib_amd import synth
+    img = synth.uniform((64, 80, 96), 9)
+    k = synth.random_psf((9, 11, 7), 3)
+    out = fc.convolve(img, k, op=fc.CORRELATE)
+    assert oracle.rel_l2(out, oracle.numpy_fft_conv(img, k, conj_kernel=True)) < TOL
