@@ -43,9 +43,15 @@ SIGNATURES = {
     "fcamd_workspace_bytes": (_ll, [_ip, _ip]),
     "fcamd_convolve3d_device": (_i, [_vp, _vp, _ip, _vp, _ip, _i, _vp, _ll, _vp]),
     "fcamd_radix_plan": (_i, [_i, _ip, _i]),
+    "fcamd_grad_kernel_workspace_bytes": (_ll, [_ip, _ip]),
+    "fcamd_conv_grad_kernel_device": (_i, [_vp, _vp, _vp, _ip, _ip, _vp, _ll, _vp]),
     "fcamd_has_static_kernel": (_i, [_i, _i]),
     "fcamd_static_lengths": (_i, [_ip, _i]),
     "fcamd_use_static_kernels": (None, [_i]),
+    "fcamd_static_variant_count": (_i, [_i, _i]),
+    "fcamd_static_variant_name": (ctypes.c_char_p, [_i, _i, _i]),
+    "fcamd_select_static_variant": (_i, [_i, _i, _i]),
+    "fcamd_selected_static_variant": (_i, [_i, _i]),
     "fcamd_pass_timing": (None, [_i]),
     "fcamd_pass_times": (_i, [ctypes.POINTER(ctypes.c_double), _ip, _i]),
     "fcamd_pass_name": (ctypes.c_char_p, [_i]),

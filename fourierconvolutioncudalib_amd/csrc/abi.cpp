@@ -280,6 +280,35 @@ int fcamd_convolve3d_device(float* d_out, const float* d_image, const int* imDim
   });
 }
 
+long long fcamd_grad_kernel_workspace_bytes(const int* imDim, const int* kernelDim) {
+  long long r = -1;
+  (void)guarded([&] { r = (long long)grad_workspace_layout(shape_from(imDim, kernelDim)).total(); });
+  return r;
+}
+
+int fcamd_conv_grad_kernel_device(float* d_dkernel, const float* d_grad_out, const float* d_image,
+                                  const int* imDim, const int* kernelDim, void* d_workspace,
+                                  long long workspace_bytes, void* stream) {
+  return guarded([&] {
+    FC_REQUIRE(d_dkernel && d_grad_out && d_image, kInvalidArgument, "null device pointer");
+    ConvShape s = shape_from(imDim, kernelDim);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const size_t need = grad_workspace_layout(s).total();
+    if (d_workspace) {
+      FC_REQUIRE(workspace_bytes >= (long long)need, kInvalidArgument, "workspace too small");
+      enqueue_grad_kernel(s, d_dkernel, d_grad_out, d_image, d_workspace, st);
+    } else {
+      int dev = 0;
+      FC_HIP_CHECK(hipGetDevice(&dev));
+      DevicePool& pool = device_pool(dev);
+      std::lock_guard<std::mutex> lk(pool.mu);
+      void* ws = pool_reserve(&pool.ws, &pool.ws_bytes, need);
+      enqueue_grad_kernel(s, d_dkernel, d_grad_out, d_image, ws, st);
+      FC_HIP_CHECK(hipStreamSynchronize(st));
+    }
+  });
+}
+
 void fcamd_pass_timing(int enable) { pass_timing_enable(enable != 0); }
 
 int fcamd_pass_times(double* ms, int* counts, int cap) {

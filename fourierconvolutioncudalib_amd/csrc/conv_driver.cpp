@@ -21,6 +21,7 @@
 #include <memory>
 #include <string>
 
+#include "fft_static.hpp"
 #include "static_kernels.hpp"
 
 namespace fcamd {
@@ -65,6 +66,51 @@ float2* twiddle_table(int dev, int L) {
   return d;
 }
 
+// Per-stage tables of the compile-time plan of L, forward then reversed
+// stage order (layout in fft_static.hpp); empty when L has no static plan.
+std::vector<float2> static_stage_table(int L) {
+  const StaticPlanData p = static_plan(L);
+  std::vector<float2> out;
+  if (p.n <= 0) return out;
+  for (int rev = 0; rev < 2; ++rev) {
+    long ns = 1;
+    for (int i = 0; i < p.n; ++i) {
+      const int r = rev ? p.r[p.n - 1 - i] : p.r[i];
+      const int rp = even_up(r);
+      const long span = ns * r;
+      for (long k = 0; k < ns; ++k)
+        for (int j = 0; j < rp; ++j) {
+          if (j >= r) {
+            out.push_back(make_float2(1.f, 0.f));
+            continue;
+          }
+          const long m = ((long)j * k) % span;
+          const double a = -2.0 * M_PI * (double)m / (double)span;
+          out.push_back(make_float2((float)std::cos(a), (float)std::sin(a)));
+        }
+      ns *= r;
+    }
+  }
+  return out;
+}
+
+std::map<std::pair<int, int>, float2*> g_stw;  // (device, L) -> stage tables
+
+float2* stage_table(int dev, int L) {
+  std::lock_guard<std::mutex> lk(g_tw_mu);
+  auto key = std::make_pair(dev, L);
+  auto it = g_stw.find(key);
+  if (it != g_stw.end()) return it->second;
+  std::vector<float2> h = static_stage_table(L);
+  float2* d = nullptr;
+  if (!h.empty()) {
+    FC_HIP_CHECK(hipMalloc(&d, sizeof(float2) * h.size()));
+    FC_HIP_CHECK(hipMemcpy(d, h.data(), sizeof(float2) * h.size(), hipMemcpyHostToDevice));
+  }
+  g_stw[key] = d;
+  return d;
+}
+
 int current_device() {
   int dev = 0;
   FC_HIP_CHECK(hipGetDevice(&dev));
@@ -82,7 +128,9 @@ FftDesc fft_desc(int L) {
   FC_REQUIRE((int)r.size() <= kMaxStages, kUnsupportedSize, "FFT plan too deep");
   d.nst = (int)r.size();
   for (int i = 0; i < d.nst; ++i) d.rad[i] = r[i];
-  d.tw = twiddle_table(current_device(), L);
+  const int dev = current_device();
+  d.tw = twiddle_table(dev, L);
+  d.stw = stage_table(dev, L);
   return d;
 }
 
@@ -209,126 +257,174 @@ void pass_timing_collect(double* ms, int* count) {
   }
 }
 
-void enqueue_convolution(const ConvShape& s, float* d_out, const float* d_image,
-                         const float* d_kernel, Op op, void* ws, hipStream_t st) {
+namespace {
+
+void ensure_kernel_attributes() {
   std::call_once(g_attr_once, [] {
     generic_kernels_allow_large_lds();
     static_kernels_allow_large_lds();
   });
-  const WorkspaceLayout wl = workspace_layout(s);
-  char* base = static_cast<char*>(ws);
-  float2* S = reinterpret_cast<float2*>(base);
-  float2* Kx = reinterpret_cast<float2*>(base + wl.spectrum);
-  float2* Kxy = reinterpret_cast<float2*>(base + wl.spectrum + wl.krows);
-  const int nb = half_bins(s.nx);
-  const long nxp = padded_bins(s.nx);
-  const FftDesc fx = fft_desc(s.nx), fy = fft_desc(s.ny), fz = fft_desc(s.nz);
+}
 
-  // K1: kernel rows
-  {
-    PassScope ps(kPassKernelRows, st);
-    XPassArgs a{};
-    a.fft = fx;
-    a.nrows = (long)s.ky * s.kz;
-    a.nbins = nb;
-    a.pairs_per_block = x_pairs_per_block(s.nx);
-    a.src = d_kernel;
-    a.dst = Kx;
-    a.src_row = s.kx;
-    a.dst_row = nxp;
-    a.kshift = 1;
-    a.klen = s.kx;
-    a.kcenter = s.cx;
-    launch_gk_xfwd(a, (a.nrows + 1) / 2, st);
-  }
-  // K2: kernel planes
-  {
-    PassScope ps(kPassKernelPlanes, st);
-    ColArgs a{};
-    a.fft = fy;
-    a.tx = col_tile(s.ny, 2);
-    a.nbins = nb;
-    a.src = Kx;
-    a.dst = Kxy;
-    a.src_outer = (long)s.ky * nxp;
-    a.src_elem = nxp;
-    a.dst_outer = (long)s.ny * nxp;
-    a.dst_elem = nxp;
-    a.nvalid = s.ky;
-    a.center = s.cy;
-    launch_gk_col(a, -1, s.kz, st);
-  }
-  // P1: image rows
-  const long nrows = (long)s.ny * s.nz;
-  {
-    PassScope ps(kPassXForward, st);
-    if (!static_xfwd(s, fx, d_image, S, st)) {
-    XPassArgs a{};
-    a.fft = fx;
-    a.nrows = nrows;
-    a.nbins = nb;
-    a.pairs_per_block = x_pairs_per_block(s.nx);
-    a.src = d_image;
-    a.dst = S;
-    a.src_row = s.nx;
-    a.dst_row = nxp;
-    launch_gk_xfwd(a, (nrows + 1) / 2, st);
-    }
-  }
-  // P2 / P4: y passes in place
+ColArgs spectrum_column_args(const ConvShape& s, const FftDesc& fy, float2* S) {
+  const long nxp = padded_bins(s.nx);
   ColArgs ya{};
   ya.fft = fy;
   ya.tx = col_tile(s.ny, 2);
-  ya.nbins = nb;
+  ya.nbins = half_bins(s.nx);
   ya.src = S;
   ya.dst = S;
   ya.src_outer = ya.dst_outer = (long)s.ny * nxp;
   ya.src_elem = ya.dst_elem = nxp;
   ya.nvalid = s.ny;
   ya.center = 0;
-  if (s.ny > 1) {
-    PassScope ps(kPassYForward, st);
-    if (!static_ycol(s, fy, S, -1, st)) launch_gk_col(ya, -1, s.nz, st);
-  }
-  // P3: fused z pass
+  return ya;
+}
+
+}  // namespace
+
+void forward_xy(const ConvShape& s, const float* src, float2* S, const float* kernel, float2* Kx,
+                float2* Kxy, hipStream_t st) {
+  ensure_kernel_attributes();
+  const int nb = half_bins(s.nx);
+  const long nxp = padded_bins(s.nx);
+  const FftDesc fx = fft_desc(s.nx), fy = fft_desc(s.ny);
+  const long nrows = (long)s.ny * s.nz;
+  // x-R2C of the image rows (and of the kernel rows, in the same launch
+  // when a compile-time kernel serves nx)
   {
-    PassScope ps(kPassZFused, st);
-    ZArgs a{};
-    a.fft = fz;
-    a.tx = col_tile(s.nz, 3);
-    a.nbins = nb;
-    a.s = S;
-    a.s_row = nxp;
-    a.s_plane = (long)s.ny * nxp;
-    a.k = Kxy;
-    a.k_row = nxp;
-    a.k_plane = (long)s.ny * nxp;
-    a.nvalid = s.kz;
-    a.center = s.cz;
-    a.conj = op == Op::kCorrelate ? 1 : 0;
-    a.scale = (float)(1.0 / (double)s.voxels());
-    if (!static_zfused(s, a, st)) launch_gk_zfused(a, s.ny, st);
+    PassScope ps(kPassXForward, st);
+    if (!static_xfwd(s, fx, src, S, kernel, Kx, st)) {
+      XPassArgs a{};
+      a.fft = fx;
+      a.nrows = nrows;
+      a.nbins = nb;
+      a.pairs_per_block = x_pairs_per_block(s.nx);
+      a.src = src;
+      a.dst = S;
+      a.src_row = s.nx;
+      a.dst_row = nxp;
+      launch_gk_xfwd(a, (nrows + 1) / 2, st);
+      if (kernel) {
+        PassScope pk(kPassKernelRows, st);
+        XPassArgs k = a;
+        k.nrows = (long)s.ky * s.kz;
+        k.src = kernel;
+        k.dst = Kx;
+        k.src_row = s.kx;
+        k.kshift = 1;
+        k.klen = s.kx;
+        k.kcenter = s.cx;
+        launch_gk_xfwd(k, (k.nrows + 1) / 2, st);
+      }
+    }
   }
+  // y-FFT of the spectrum (and kernel rows -> kernel planes)
+  {
+    PassScope ps(kPassYForward, st);
+    if (!static_ycol(s, fy, S, -1, kernel ? Kx : nullptr, Kxy, st)) {
+      if (s.ny > 1) launch_gk_col(spectrum_column_args(s, fy, S), -1, s.nz, st);
+      if (kernel) {
+        PassScope pk(kPassKernelPlanes, st);
+        ColArgs a{};
+        a.fft = fy;
+        a.tx = col_tile(s.ny, 2);
+        a.nbins = nb;
+        a.src = Kx;
+        a.dst = Kxy;
+        a.src_outer = (long)s.ky * nxp;
+        a.src_elem = nxp;
+        a.dst_outer = (long)s.ny * nxp;
+        a.dst_elem = nxp;
+        a.nvalid = s.ky;
+        a.center = s.cy;
+        launch_gk_col(a, -1, s.kz, st);
+      }
+    }
+  }
+}
+
+void z_pass(const ConvShape& s, float2* S, const float2* operand, bool dense, bool conj,
+            hipStream_t st) {
+  ensure_kernel_attributes();
+  const long nxp = padded_bins(s.nx);
+  PassScope ps(kPassZFused, st);
+  ZArgs a{};
+  a.fft = fft_desc(s.nz);
+  a.tx = col_tile(s.nz, 3);
+  a.nbins = half_bins(s.nx);
+  a.s = S;
+  a.s_row = nxp;
+  a.s_plane = (long)s.ny * nxp;
+  a.k = operand;
+  a.k_row = nxp;
+  a.k_plane = (long)s.ny * nxp;
+  a.nvalid = dense ? s.nz : s.kz;
+  a.center = dense ? 0 : s.cz;
+  a.conj = conj ? 1 : 0;
+  a.scale = (float)(1.0 / (double)s.voxels());
+  if (!static_zfused(s, a, st)) launch_gk_zfused(a, s.ny, st);
+}
+
+void inverse_yx(const ConvShape& s, float2* S, float* dst, hipStream_t st) {
+  ensure_kernel_attributes();
+  const FftDesc fx = fft_desc(s.nx), fy = fft_desc(s.ny);
   if (s.ny > 1) {
     PassScope ps(kPassYInverse, st);
-    if (!static_ycol(s, fy, S, +1, st)) launch_gk_col(ya, +1, s.nz, st);
+    if (!static_ycol(s, fy, S, +1, nullptr, nullptr, st))
+      launch_gk_col(spectrum_column_args(s, fy, S), +1, s.nz, st);
   }
-  // P5: back to real rows
-  {
-    PassScope ps(kPassXInverse, st);
-    if (!static_xinv(s, fx, S, d_out, st)) {
+  PassScope ps(kPassXInverse, st);
+  if (!static_xinv(s, fx, S, dst, st)) {
+    const long nrows = (long)s.ny * s.nz;
     XPassArgs a{};
     a.fft = fx;
     a.nrows = nrows;
-    a.nbins = nb;
+    a.nbins = half_bins(s.nx);
     a.pairs_per_block = x_pairs_per_block(s.nx);
     a.csrc = S;
-    a.rdst = d_out;
-    a.src_row = nxp;
+    a.rdst = dst;
+    a.src_row = padded_bins(s.nx);
     a.dst_row = s.nx;
     launch_gk_xinv(a, (nrows + 1) / 2, st);
-    }
   }
+}
+
+void enqueue_convolution(const ConvShape& s, float* d_out, const float* d_image,
+                         const float* d_kernel, Op op, void* ws, hipStream_t st) {
+  const WorkspaceLayout wl = workspace_layout(s);
+  char* base = static_cast<char*>(ws);
+  float2* S = reinterpret_cast<float2*>(base);
+  float2* Kx = reinterpret_cast<float2*>(base + wl.spectrum);
+  float2* Kxy = reinterpret_cast<float2*>(base + wl.spectrum + wl.krows);
+  forward_xy(s, d_image, S, d_kernel, Kx, Kxy, st);
+  z_pass(s, S, Kxy, false, op == Op::kCorrelate, st);
+  inverse_yx(s, S, d_out, st);
+  FC_HIP_CHECK(hipGetLastError());
+}
+
+GradWorkspaceLayout grad_workspace_layout(const ConvShape& s) {
+  const size_t nxp = (size_t)padded_bins(s.nx);
+  GradWorkspaceLayout w;
+  w.spectrum = align256(sizeof(float2) * nxp * s.ny * s.nz);
+  w.corr = align256(sizeof(float) * (size_t)s.voxels());
+  return w;
+}
+
+void enqueue_grad_kernel(const ConvShape& s, float* d_dkernel, const float* d_grad_out,
+                         const float* d_image, void* ws, hipStream_t st) {
+  const GradWorkspaceLayout wl = grad_workspace_layout(s);
+  char* base = static_cast<char*>(ws);
+  float2* G = reinterpret_cast<float2*>(base);
+  float2* I = reinterpret_cast<float2*>(base + wl.spectrum);
+  float* U = reinterpret_cast<float*>(base + 2 * wl.spectrum);
+  // u = IFFT(FFT(g) * conj(FFT(im))) / N, the circular cross-correlation
+  // u[d] = sum_i g[i] im[i - d]; dk[j] = u[(j - c) mod n]
+  forward_xy(s, d_grad_out, G, nullptr, nullptr, nullptr, st);
+  forward_xy(s, d_image, I, nullptr, nullptr, nullptr, st);
+  z_pass(s, G, I, true, true, st);
+  inverse_yx(s, G, U, st);
+  launch_gather_kernel_support(s, U, d_dkernel, st);
   FC_HIP_CHECK(hipGetLastError());
 }
 

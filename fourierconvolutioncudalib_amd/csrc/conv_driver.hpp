@@ -66,6 +66,32 @@ void pass_timing_collect(double* ms, int* count);
 void enqueue_convolution(const ConvShape& s, float* d_out, const float* d_image,
                          const float* d_kernel, Op op, void* ws, hipStream_t stream);
 
+// Building blocks (all asynchronous on `stream`):
+//   forward_xy  real volume -> half spectrum S (x-R2C, y-FFT); with `kernel`
+//               also the compact kernel's rows Kx and planes Kxy
+//   z_pass      S <- IFFTz(FFTz(S) * op(FFTz(operand))) / N, operand the
+//               compact kernel planes (dense = false) or a second spectrum
+//   inverse_yx  S -> real volume (inverse y-FFT, x-C2R), unnormalised
+void forward_xy(const ConvShape& s, const float* src, float2* S, const float* kernel, float2* Kx,
+                float2* Kxy, hipStream_t stream);
+void z_pass(const ConvShape& s, float2* S, const float2* operand, bool dense, bool conj,
+            hipStream_t stream);
+void inverse_yx(const ConvShape& s, float2* S, float* dst, hipStream_t stream);
+
+// Gradient of sum(g * conv(im, k)) with respect to the kernel taps:
+// dk[j] = sum_i g[i] im[(i - (j - c)) mod n] on the kernel support.
+struct GradWorkspaceLayout {
+  size_t spectrum;  // two spectra of the image's size
+  size_t corr;      // image-sized float correlation volume
+  size_t total() const { return 2 * spectrum + corr; }
+};
+GradWorkspaceLayout grad_workspace_layout(const ConvShape& s);
+void enqueue_grad_kernel(const ConvShape& s, float* d_dkernel, const float* d_grad_out,
+                         const float* d_image, void* ws, hipStream_t stream);
+// dk[jz][jy][jx] = u[(jz-cz) mod nz][(jy-cy) mod ny][(jx-cx) mod nx]
+void launch_gather_kernel_support(const ConvShape& s, const float* u, float* dk,
+                                  hipStream_t stream);
+
 // Per-device pooled resources for callers that pass no workspace.
 struct DevicePool {
   std::mutex mu;

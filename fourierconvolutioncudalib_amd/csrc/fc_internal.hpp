@@ -69,7 +69,8 @@ struct FftDesc {
   int L;
   int nst;
   int rad[kMaxStages];
-  const float2* tw;
+  const float2* tw;   // runtime-length kernels
+  const float2* stw;  // per-stage tables of the compile-time plan (or null)
 };
 
 }  // namespace fcamd

@@ -25,23 +25,40 @@ struct StaticPlanData {
   int r[16] = {};
 };
 
-// Balanced radix plan for a 7-smooth length: powers of two as 8s with the
-// remainder as one 4 (2^(3k+2)) or two 4s (2^(3k+1)); then 3s, 5s, 7s.
+// Radix plan for a 7-smooth length: the power-of-two part in the fewest
+// stages of radix <= 16 with the bits spread evenly (256 = 16*16,
+// 512 = 8*8*8, 1024 = 16*8*8, 128 = 16*8), then 3s, 5s and 7s.
 constexpr StaticPlanData static_plan(int L) {
   StaticPlanData p{};
   int a = 0;
   int n = L;
   while (n % 2 == 0) { n /= 2; ++a; }
-  int eights = a / 3, rem = a % 3;
-  if (rem == 1 && eights > 0) { --eights; p.r[p.n++] = 4; p.r[p.n++] = 4; }
-  for (int i = 0; i < eights; ++i) p.r[p.n++] = 8;
-  if (rem == 2) p.r[p.n++] = 4;
-  if (rem == 1 && a == 1) p.r[p.n++] = 2;
+  if (a > 0) {
+    const int st = (a + 3) / 4;
+    const int lo = a / st, extra = a % st;
+    for (int i = 0; i < st; ++i) p.r[p.n++] = 1 << (lo + (i < extra ? 1 : 0));
+  }
   while (n % 3 == 0) { p.r[p.n++] = 3; n /= 3; }
   while (n % 5 == 0) { p.r[p.n++] = 5; n /= 5; }
   while (n % 7 == 0) { p.r[p.n++] = 7; n /= 7; }
   if (n != 1) p.n = -1;  // not 7-smooth: no static plan
   return p;
+}
+
+// Per-stage twiddle tables of a static plan, stored for the forward stage
+// order and then for the reversed order: stage s holds Ns(s) rows of
+// even(R) float2, row k = exp(-2 pi i r k / (Ns R)) for r = 0..R-1, so a
+// thread fetches its butterfly's twiddles with R/2 aligned 16-byte loads.
+constexpr int even_up(int r) { return (r + 1) / 2 * 2; }
+
+constexpr int static_tw_size(const StaticPlanData& p, bool rev) {
+  int total = 0, ns = 1;
+  for (int i = 0; i < p.n; ++i) {
+    const int r = rev ? p.r[p.n - 1 - i] : p.r[i];
+    total += ns * even_up(r);
+    ns *= r;
+  }
+  return total;
 }
 
 constexpr int plan_rmax(const StaticPlanData& p) {
@@ -71,15 +88,21 @@ struct PlanOf {
     for (int s = 0; s < S; ++s) m = E(s) > m ? E(s) : m;
     return m;
   }
+  // offset (float2 units) of stage s's table in the combined table
+  static constexpr int TWOFF(int s) {
+    int off = REV ? static_tw_size(data, false) : 0;
+    for (int i = 0; i < s; ++i) off += Ns(i) * even_up(R(i));
+    return off;
+  }
 };
 
 // One stage's butterflies on the thread's registers (inputs already loaded
 // at positions b + r*M, outputs left in place).
 template <int L, bool REV, int DIR, int s, int MAXE>
-__device__ __forceinline__ void sfft_compute(float2 (&v)[MAXE], int t, const float2* __restrict__ tw) {
+__device__ __forceinline__ void sfft_compute(float2 (&v)[MAXE], int t, const float2* __restrict__ stw) {
   using P = PlanOf<L, REV>;
   constexpr int R = P::R(s), Ns = P::Ns(s), M = P::M(s), NB = P::NB(s);
-  constexpr int twstep = L / (Ns * R);
+  constexpr int RP = even_up(R);
 #pragma unroll
   for (int i = 0; i < NB; ++i) {
     const int b = t + i * P::T;
@@ -87,8 +110,16 @@ __device__ __forceinline__ void sfft_compute(float2 (&v)[MAXE], int t, const flo
     float2* x = &v[i * R];
     if (Ns > 1) {
       const int k = b % Ns;
+      const float4* q = reinterpret_cast<const float4*>(stw + P::TWOFF(s) + k * RP);
+      float2 w[RP];
 #pragma unroll
-      for (int r = 1; r < R; ++r) x[r] = cmul(x[r], twiddle<DIR>(tw, r * k * twstep));
+      for (int r2 = 0; r2 < RP / 2; ++r2) {
+        const float4 p = q[r2];
+        w[2 * r2] = make_float2(p.x, p.y);
+        w[2 * r2 + 1] = make_float2(p.z, p.w);
+      }
+#pragma unroll
+      for (int r = 1; r < R; ++r) x[r] = DIR < 0 ? cmul(x[r], w[r]) : cmulc(x[r], w[r]);
     }
     Dft<R, DIR>::run(x);
   }

@@ -52,6 +52,25 @@ struct ZArgs {
   float scale;          // 1 / (nx*ny*nz)
 };
 
+// A pass launch that also carries the compact kernel's rows / planes as
+// extra workgroups (they share the transform length with the image).
+struct XPass2 {
+  XPassArgs m;      // image rows
+  XPassArgs k;      // kernel rows (kshift)
+  long m_blocks;    // workgroups of the image part
+  long k_blocks;    // workgroups of the kernel part (they come first)
+};
+struct Col2 {
+  ColArgs m;        // image columns
+  ColArgs k;        // kernel planes
+  long m_outer;     // image tile rows (grid rows of the image part)
+  long k_outer;     // kernel tile rows (0: no kernel part)
+  // filled by the launcher for the chosen tile width:
+  int ntx;          // bin tiles per row
+  long m_tiles;     // tiles of the image part (after the kernel's)
+  long ntiles;      // total tiles
+};
+
 void launch_gk_xfwd(const XPassArgs& a, long npairs, hipStream_t st);
 void launch_gk_xinv(const XPassArgs& a, long npairs, hipStream_t st);
 void launch_gk_col(const ColArgs& a, int dir, long nouter, hipStream_t st);

@@ -26,6 +26,35 @@ stream_copy(float4* __restrict__ dst, const float4* __restrict__ src, long n) {
     dst[i] = src[i];
 }
 
+// dk[jz][jy][jx] = u[(jz-cz) mod nz][(jy-cy) mod ny][(jx-cx) mod nx]
+__global__ void __launch_bounds__(256)
+gather_kernel_support(const float* __restrict__ u, float* __restrict__ dk, int nx, int ny, int nz,
+                      int kx, int ky, int kz, int cx, int cy, int cz) {
+  const long K = (long)kx * ky * kz;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < K; i += (long)gridDim.x * blockDim.x) {
+    const int jx = (int)(i % kx);
+    const int jy = (int)((i / kx) % ky);
+    const int jz = (int)(i / ((long)kx * ky));
+    int x = jx - cx, y = jy - cy, z = jz - cz;
+    if (x < 0) x += nx;
+    if (y < 0) y += ny;
+    if (z < 0) z += nz;
+    dk[i] = u[((long)z * ny + y) * nx + x];
+  }
+}
+
+}  // namespace fcamd
+
+#include "conv_driver.hpp"
+
+namespace fcamd {
+void launch_gather_kernel_support(const ConvShape& s, const float* u, float* dk, hipStream_t st) {
+  const long K = (long)s.kx * s.ky * s.kz;
+  long blocks = (K + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(gather_kernel_support, dim3((unsigned)blocks), dim3(256), 0, st, u, dk, s.nx,
+                     s.ny, s.nz, s.kx, s.ky, s.kz, s.cx, s.cy, s.cz);
+}
 }  // namespace fcamd
 
 using namespace fcamd;

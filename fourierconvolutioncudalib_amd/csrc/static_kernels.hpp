@@ -8,11 +8,16 @@
 
 namespace fcamd {
 
+// x-R2C of the image rows; with `kernel` non-null the same launch also
+// transforms the compact kernel's rows into `krows`.
 bool static_xfwd(const ConvShape& s, const FftDesc& fx, const float* src, float2* dst,
-                 hipStream_t st);
+                 const float* kernel, float2* krows, hipStream_t st);
 bool static_xinv(const ConvShape& s, const FftDesc& fx, const float2* src, float* dst,
                  hipStream_t st);
-bool static_ycol(const ConvShape& s, const FftDesc& fy, float2* S, int dir, hipStream_t st);
+// y-FFT of the spectrum in place; with `krows` non-null the same launch also
+// turns the kernel rows into the kernel planes `kplanes`.
+bool static_ycol(const ConvShape& s, const FftDesc& fy, float2* S, int dir, const float2* krows,
+                 float2* kplanes, hipStream_t st);
 bool static_zfused(const ConvShape& s, const ZArgs& a, hipStream_t st);
 void static_kernels_allow_large_lds();
 

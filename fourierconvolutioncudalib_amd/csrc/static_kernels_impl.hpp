@@ -1,6 +1,18 @@
 // static_kernels_impl.hpp -- compile-time-length pass kernels built on
-// fft_static.hpp.  Instantiated per length by static_inst.hip; the driver
-// reaches them through the registry in static_kernels.hip.
+// fft_static.hpp, in several variants per pass.  Instantiated per length by
+// static_inst.hip; the driver reaches them through the registry in
+// static_kernels.hip.
+//
+// Variants:
+//   x forward  "p<P>"            P row pairs per workgroup
+//   x inverse  "p<P>"
+//   column     "t<TX>"           one TX-bin tile per workgroup
+//              "t<TX>-pipe"      persistent workgroups; the next tile's loads
+//                                are issued before the current tile's FFT
+//   fused z    "t<TX>-fft"       kernel column by the same FFT as the image
+//              "t<TX>-dft"       kernel spectrum by direct DFT over its kz
+//                                taps straight into the last-stage registers
+//              "t<TX>-dft-pipe"  the same, persistent and pipelined
 #pragma once
 
 #include "fft_static.hpp"
@@ -9,33 +21,26 @@
 
 namespace fcamd {
 
-// bins per column tile: 16 (one 128-byte line of float2) when the
-// workgroup stays within 1024 threads and 64 KiB of LDS, else 8, 4, ...
 template <int L>
-constexpr int col_tile_static() {
-  constexpr int T = PlanOf<L, false>::T;
-  int tx = 16;
-  while (tx > 1 && (tx * T > 1024 || (long)tx * L * 8 > 65536)) tx /= 2;
+constexpr int threads_per_transform() {
+  return PlanOf<L, false>::T;
+}
+
+// largest tile <= want that keeps a workgroup within 1024 threads and
+// 64 KiB of LDS
+template <int L, int WANT>
+constexpr int fit_tile() {
+  int tx = WANT;
+  while (tx > 1 && (tx * threads_per_transform<L>() > 1024 || (long)tx * L * 8 > 65536)) tx /= 2;
   return tx;
 }
 
-// row pairs per workgroup of the x passes (about 256 threads)
-template <int L>
-constexpr int x_pairs_static() {
-  constexpr int T = PlanOf<L, false>::T;
-  int p = 256 / T;
-  if (p < 1) p = 1;
-  while (p > 1 && (long)p * (L + L / 8 + 1) * 8 > 65536) p /= 2;
+template <int L, int WANT>
+constexpr int fit_pairs() {
+  int p = WANT;
+  while (p > 1 && (p * threads_per_transform<L>() > 1024 || (long)p * (L + L / 8 + 1) * 8 > 65536))
+    p /= 2;
   return p;
-}
-
-template <int L>
-constexpr int x_threads_static() {
-  return x_pairs_static<L>() * PlanOf<L, false>::T;
-}
-template <int L>
-constexpr int col_threads_static() {
-  return col_tile_static<L>() * PlanOf<L, false>::T;
 }
 
 // padded LDS row of the x passes: one float2 of padding per 8 keeps the
@@ -47,29 +52,41 @@ struct XRow {
 };
 
 // ---------------------------------------------------------------------------
-// x pass forward: two real rows per complex FFT (see kernels_generic.hip)
+// x pass forward: two real rows per complex FFT (see kernels_generic.hip).
+// The first k_blocks workgroups transform the compact kernel's rows, placed
+// circularly so tap x lands at (x - kx/2) mod nx.
 // ---------------------------------------------------------------------------
-template <int L>
-__global__ void __launch_bounds__(x_threads_static<L>())
-sk_xfwd(XPassArgs a) {
+template <int L, int PAIRS>
+__global__ void __launch_bounds__(PAIRS * threads_per_transform<L>())
+sk_xfwd(XPass2 a2) {
   using P = PlanOf<L, false>;
-  constexpr int T = P::T, PAIRS = x_pairs_static<L>(), LP = XRow<L>::LP;
+  constexpr int T = P::T, LP = XRow<L>::LP;
   __shared__ float2 lds[PAIRS * LP];
+  // the kernel's rows take the first k_blocks workgroups so they run
+  // alongside the image rows instead of trailing them
+  const bool main = (long)blockIdx.x >= a2.k_blocks;
+  const XPassArgs& a = main ? a2.m : a2.k;
+  const long blk = main ? (long)blockIdx.x - a2.k_blocks : (long)blockIdx.x;
   const int t = threadIdx.x % T, c = threadIdx.x / T;
-  const long ra = 2 * ((long)blockIdx.x * PAIRS + c), rb = ra + 1;
+  const long ra = 2 * (blk * PAIRS + c), rb = ra + 1;
   const bool oka = ra < a.nrows, okb = rb < a.nrows;
   const float* pa = a.src + ra * a.src_row;
   const float* pb = a.src + rb * a.src_row;
+  const int kc = a.kcenter, kl = a.klen;
   float2 v[P::MAXE()];
   sfft_for_inputs<L, false>(t, [&](int slot, int pos) {
+    int x = pos + kc;
+    if (x >= L) x -= L;
     float xa = 0.f, xb = 0.f;
-    if (oka) xa = pa[pos];
-    if (okb) xb = pb[pos];
+    if (x < kl) {
+      if (oka) xa = pa[x];
+      if (okb) xb = pb[x];
+    }
     v[slot] = make_float2(xa, xb);
   });
   float2* row = lds + c * LP;
   auto addr = [&](int pos) { return XRow<L>::at(pos); };
-  sfft_run<L, false, -1>(v, t, row, addr, a.fft.tw);
+  sfft_run<L, false, -1>(v, t, row, addr, a.fft.stw);
   sfft_for_outputs<L, false>(t, [&](int slot, int pos) { row[addr(pos)] = v[slot]; });
   __syncthreads();
   float2* da = a.dst + ra * a.dst_row;
@@ -85,11 +102,11 @@ sk_xfwd(XPassArgs a) {
 // ---------------------------------------------------------------------------
 // x pass inverse (C2R), two rows per complex FFT
 // ---------------------------------------------------------------------------
-template <int L>
-__global__ void __launch_bounds__(x_threads_static<L>())
+template <int L, int PAIRS>
+__global__ void __launch_bounds__(PAIRS * threads_per_transform<L>())
 sk_xinv(XPassArgs a) {
   using P = PlanOf<L, false>;
-  constexpr int T = P::T, PAIRS = x_pairs_static<L>(), LP = XRow<L>::LP;
+  constexpr int T = P::T, LP = XRow<L>::LP;
   __shared__ float2 lds[PAIRS * LP];
   const int t = threadIdx.x % T, c = threadIdx.x / T;
   const long ra = 2 * ((long)blockIdx.x * PAIRS + c), rb = ra + 1;
@@ -116,7 +133,7 @@ sk_xinv(XPassArgs a) {
   });
   float2* row = lds + c * LP;
   auto addr = [&](int pos) { return XRow<L>::at(pos); };
-  sfft_run<L, false, +1>(v, t, row, addr, a.fft.tw);
+  sfft_run<L, false, +1>(v, t, row, addr, a.fft.stw);
   float* oa = a.rdst + ra * a.dst_row;
   float* ob = a.rdst + rb * a.dst_row;
   sfft_for_outputs<L, false>(t, [&](int slot, int pos) {
@@ -126,122 +143,366 @@ sk_xinv(XPassArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// column pass (y): tile of TX bins x L positions
+// column pass (y): tiles of TX bins x L positions over a 1D tile index
+// (image tiles first, then the kernel planes' tiles)
 // ---------------------------------------------------------------------------
-template <int L, int DIR>
-__global__ void __launch_bounds__(col_threads_static<L>())
-sk_col(ColArgs a) {
+template <int L, int TX>
+struct ColTile {
   using P = PlanOf<L, false>;
-  constexpr int TX = col_tile_static<L>();
+  static constexpr int MAXE = P::MAXE();
+  const ColArgs* a;
+  long base_src, base_dst;
+  bool ok;
+
+  __device__ ColTile(const Col2& a2, long tile, int c) {
+    // kernel-plane tiles first, then the image's
+    const long ktiles = a2.ntiles - a2.m_tiles;
+    const bool main = tile >= ktiles;
+    a = main ? &a2.m : &a2.k;
+    const long tl = main ? tile - ktiles : tile;
+    const long o = tl / a2.ntx;
+    const int x = (int)(tl - o * a2.ntx) * TX + c;
+    ok = x < a->nbins;
+    base_src = o * a->src_outer + x;
+    base_dst = o * a->dst_outer + x;
+  }
+  __device__ void load(int t, float2 (&v)[MAXE]) const {
+    const float2* src = a->src + base_src;
+    const int center = a->center, nvalid = a->nvalid;
+    const long es = a->src_elem;
+    const bool okk = ok;
+    sfft_for_inputs<L, false>(t, [&](int slot, int pos) {
+      int j = pos + center;
+      if (j >= L) j -= L;
+      float2 val = make_float2(0.f, 0.f);
+      if (okk && j < nvalid) val = src[(long)j * es];
+      v[slot] = val;
+    });
+  }
+  __device__ void store(int t, const float2 (&v)[MAXE]) const {
+    if (!ok) return;
+    float2* dst = a->dst + base_dst;
+    const long es = a->dst_elem;
+    sfft_for_outputs<L, false>(t, [&](int slot, int pos) { dst[(long)pos * es] = v[slot]; });
+  }
+};
+
+template <int L, int TX, int DIR>
+__global__ void __launch_bounds__(TX * threads_per_transform<L>())
+sk_col(Col2 a2) {
+  using CT = ColTile<L, TX>;
   __shared__ float2 lds[L * TX];
   const int c = threadIdx.x % TX, t = threadIdx.x / TX;
-  const int x = blockIdx.x * TX + c;
-  const bool ok = x < a.nbins;
-  const long o = blockIdx.y;
-  const float2* src = a.src + o * a.src_outer + x;
-  float2 v[P::MAXE()];
-  sfft_for_inputs<L, false>(t, [&](int slot, int pos) {
-    int j = pos + a.center;
-    if (j >= L) j -= L;
-    float2 val = make_float2(0.f, 0.f);
-    if (ok && j < a.nvalid) val = src[(long)j * a.src_elem];
-    v[slot] = val;
-  });
+  const CT tile(a2, blockIdx.x, c);
+  float2 v[CT::MAXE];
+  tile.load(t, v);
   auto addr = [&](int pos) { return pos * TX + c; };
-  sfft_run<L, false, DIR>(v, t, lds, addr, a.fft.tw);
-  float2* dst = a.dst + o * a.dst_outer + x;
-  if (ok) sfft_for_outputs<L, false>(t, [&](int slot, int pos) { dst[(long)pos * a.dst_elem] = v[slot]; });
+  sfft_run<L, false, DIR>(v, t, lds, addr, tile.a->fft.stw);
+  tile.store(t, v);
+}
+
+template <int L, int TX, int DIR>
+__global__ void __launch_bounds__(TX * threads_per_transform<L>())
+sk_col_pipe(Col2 a2) {
+  using CT = ColTile<L, TX>;
+  __shared__ float2 lds[L * TX];
+  const int c = threadIdx.x % TX, t = threadIdx.x / TX;
+  auto addr = [&](int pos) { return pos * TX + c; };
+  long tile = blockIdx.x;
+  float2 v[CT::MAXE], nv[CT::MAXE];
+  if (tile < a2.ntiles) CT(a2, tile, c).load(t, v);
+  for (; tile < a2.ntiles; tile += gridDim.x) {
+    const long next = tile + gridDim.x;
+    if (next < a2.ntiles) CT(a2, next, c).load(t, nv);
+    const CT cur(a2, tile, c);
+    sfft_run<L, false, DIR>(v, t, lds, addr, cur.a->fft.stw);
+    cur.store(t, v);
+#pragma unroll
+    for (int i = 0; i < CT::MAXE; ++i) v[i] = nv[i];
+  }
 }
 
 // ---------------------------------------------------------------------------
-// fused z pass: forward image column, forward kernel column, product,
-// inverse (reversed plan) -- all in registers between the FFTs
+// fused z pass
 // ---------------------------------------------------------------------------
-template <int L>
-__global__ void __launch_bounds__(col_threads_static<L>())
-sk_zfused(ZArgs a) {
+// Kernel spectrum at the thread's last-stage output positions f = b + M*r
+// (M = L/R, R = last radix) by direct DFT over the kz taps of the column:
+//   K[f] = sum_j k_j W^(f (j - cz)),  W = exp(-2 pi i / L)
+// With acc_q = sum_{j = q mod R} k_j W^(b (j - cz)) this is
+//   K[b + M r] = W_R^(-r cz) * DFT_R(acc)[r].
+template <int L, int MAXE>
+__device__ __forceinline__ void kernel_spectrum_direct(float2 (&w)[MAXE], int t, bool ok,
+                                                       const float2* __restrict__ kcol, long kplane,
+                                                       int kz, int cz,
+                                                       const float2* __restrict__ tw) {
   using P = PlanOf<L, false>;
-  constexpr int TX = col_tile_static<L>();
-  __shared__ float2 lds[L * TX];
-  const int c = threadIdx.x % TX, t = threadIdx.x / TX;
-  const int x = blockIdx.x * TX + c;
-  const bool ok = x < a.nbins;
-  const long y = blockIdx.y;
-  float2* col = a.s + y * a.s_row + x;
-  const float2* kcol = a.k + y * a.k_row + x;
-  auto addr = [&](int pos) { return pos * TX + c; };
+  constexpr int s = P::S - 1;
+  constexpr int R = P::R(s), M = P::M(s), NB = P::NB(s);
+#pragma unroll
+  for (int i = 0; i < NB * R; ++i) w[i] = make_float2(0.f, 0.f);
+  // twiddle exponent e_i = b_i * (j - cz) mod L, advanced by b_i per tap
+  int e[NB];
+  int bb[NB];
+#pragma unroll
+  for (int i = 0; i < NB; ++i) {
+    bb[i] = t + i * P::T;
+    e[i] = (int)(((long)bb[i] * (L - cz)) % L);
+  }
+  for (int j0 = 0; j0 < kz; j0 += R) {
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const int j = j0 + q;
+      if (j < kz) {
+        float2 kv = make_float2(0.f, 0.f);
+        if (ok) kv = kcol[(long)j * kplane];
+#pragma unroll
+        for (int i = 0; i < NB; ++i) {
+          const float2 wt = tw[e[i]];
+          w[i * R + q] = make_float2(w[i * R + q].x + kv.x * wt.x - kv.y * wt.y,
+                                     w[i * R + q].y + kv.x * wt.y + kv.y * wt.x);
+          e[i] += bb[i];
+          if (e[i] >= L) e[i] -= L;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < NB; ++i) {
+    Dft<R, -1>::run(&w[i * R]);
+    // W_R^(-r cz) = conj(tw[M * ((r cz) mod R)])
+#pragma unroll
+    for (int r = 1; r < R; ++r) w[i * R + r] = cmulc(w[i * R + r], tw[M * ((r * cz) % R)]);
+  }
+}
 
-  float2 v[P::MAXE()];
-  sfft_for_inputs<L, false>(t, [&](int slot, int pos) {
-    float2 val = make_float2(0.f, 0.f);
-    if (ok) val = col[(long)pos * a.s_plane];
-    v[slot] = val;
-  });
-  float2 w[P::MAXE()];
-  sfft_for_inputs<L, false>(t, [&](int slot, int pos) {
-    int j = pos + a.center;
-    if (j >= L) j -= L;
-    float2 val = make_float2(0.f, 0.f);
-    if (ok && j < a.nvalid) val = kcol[(long)j * a.k_plane];
-    w[slot] = val;
-  });
-  // every sfft_run ends with a barrier after its last LDS read, so the
-  // next transform may reuse the tile
-  sfft_run<L, false, -1>(v, t, lds, addr, a.fft.tw);
-  sfft_run<L, false, -1>(w, t, lds, addr, a.fft.tw);
-  const float sc = a.scale;
-  const bool cj = a.conj != 0;
+template <int L, int TX>
+struct ZTile {
+  using P = PlanOf<L, false>;
+  static constexpr int MAXE = P::MAXE();
+  float2* col;
+  const float2* kcol;
+  bool ok;
+  __device__ ZTile(const ZArgs& a, long tile, int ntx, int c) {
+    const long y = tile / ntx;
+    const int x = (int)(tile - y * ntx) * TX + c;
+    ok = x < a.nbins;
+    col = a.s + y * a.s_row + x;
+    kcol = a.k + y * a.k_row + x;
+  }
+  __device__ void load(const ZArgs& a, int t, float2 (&v)[MAXE]) const {
+    const long sp = a.s_plane;
+    const bool okk = ok;
+    float2* cc = col;
+    sfft_for_inputs<L, false>(t, [&](int slot, int pos) {
+      float2 val = make_float2(0.f, 0.f);
+      if (okk) val = cc[(long)pos * sp];
+      v[slot] = val;
+    });
+  }
+  __device__ void store(const ZArgs& a, int t, const float2 (&v)[MAXE]) const {
+    if (!ok) return;
+    const long sp = a.s_plane;
+    float2* cc = col;
+    sfft_for_outputs<L, true>(t, [&](int slot, int pos) { cc[(long)pos * sp] = v[slot]; });
+  }
+};
+
+// product of the forward image spectrum (v) with the kernel spectrum (w)
+template <int L, int MAXE>
+__device__ __forceinline__ void zproduct(float2 (&v)[MAXE], const float2 (&w)[MAXE], int t,
+                                         bool cj, float sc) {
   sfft_for_outputs<L, false>(t, [&](int slot, int) {
     const float2 p = cj ? cmulc(v[slot], w[slot]) : cmul(v[slot], w[slot]);
     v[slot] = cscale(p, sc);
   });
-  if constexpr (P::S > 1) __syncthreads();
-  sfft_run<L, true, +1>(v, t, lds, addr, a.fft.tw);
-  if (ok) sfft_for_outputs<L, true>(t, [&](int slot, int pos) { col[(long)pos * a.s_plane] = v[slot]; });
+}
+
+template <int L, int TX>
+__global__ void __launch_bounds__(TX * threads_per_transform<L>())
+sk_zfused_fft(ZArgs a, int ntx) {
+  using ZT = ZTile<L, TX>;
+  __shared__ float2 lds[L * TX];
+  const int c = threadIdx.x % TX, t = threadIdx.x / TX;
+  const ZT tile(a, blockIdx.x, ntx, c);
+  auto addr = [&](int pos) { return pos * TX + c; };
+  float2 v[ZT::MAXE];
+  tile.load(a, t, v);
+  float2 w[ZT::MAXE];
+  {
+    const long kp = a.k_plane;
+    const int center = a.center, nvalid = a.nvalid;
+    const bool ok = tile.ok;
+    const float2* kc = tile.kcol;
+    sfft_for_inputs<L, false>(t, [&](int slot, int pos) {
+      int j = pos + center;
+      if (j >= L) j -= L;
+      float2 val = make_float2(0.f, 0.f);
+      if (ok && j < nvalid) val = kc[(long)j * kp];
+      w[slot] = val;
+    });
+  }
+  // every sfft_run ends with a barrier after its last LDS read, so the
+  // next transform may reuse the tile
+  sfft_run<L, false, -1>(v, t, lds, addr, a.fft.stw);
+  sfft_run<L, false, -1>(w, t, lds, addr, a.fft.stw);
+  zproduct<L>(v, w, t, a.conj != 0, a.scale);
+  sfft_run<L, true, +1>(v, t, lds, addr, a.fft.stw);
+  tile.store(a, t, v);
+}
+
+template <int L, int TX>
+__global__ void __launch_bounds__(TX * threads_per_transform<L>())
+sk_zfused_dft(ZArgs a, int ntx) {
+  using ZT = ZTile<L, TX>;
+  __shared__ float2 lds[L * TX];
+  const int c = threadIdx.x % TX, t = threadIdx.x / TX;
+  const ZT tile(a, blockIdx.x, ntx, c);
+  auto addr = [&](int pos) { return pos * TX + c; };
+  float2 v[ZT::MAXE];
+  tile.load(a, t, v);
+  sfft_run<L, false, -1>(v, t, lds, addr, a.fft.stw);
+  float2 w[ZT::MAXE];
+  kernel_spectrum_direct<L>(w, t, tile.ok, tile.kcol, a.k_plane, a.nvalid, a.center, a.fft.tw);
+  zproduct<L>(v, w, t, a.conj != 0, a.scale);
+  sfft_run<L, true, +1>(v, t, lds, addr, a.fft.stw);
+  tile.store(a, t, v);
+}
+
+template <int L, int TX>
+__global__ void __launch_bounds__(TX * threads_per_transform<L>())
+sk_zfused_dft_pipe(ZArgs a, int ntx, long ntiles) {
+  using ZT = ZTile<L, TX>;
+  __shared__ float2 lds[L * TX];
+  const int c = threadIdx.x % TX, t = threadIdx.x / TX;
+  auto addr = [&](int pos) { return pos * TX + c; };
+  long tile = blockIdx.x;
+  float2 v[ZT::MAXE], nv[ZT::MAXE];
+  if (tile < ntiles) ZT(a, tile, ntx, c).load(a, t, v);
+  for (; tile < ntiles; tile += gridDim.x) {
+    const long next = tile + gridDim.x;
+    if (next < ntiles) ZT(a, next, ntx, c).load(a, t, nv);
+    const ZT cur(a, tile, ntx, c);
+    sfft_run<L, false, -1>(v, t, lds, addr, a.fft.stw);
+    float2 w[ZT::MAXE];
+    kernel_spectrum_direct<L>(w, t, cur.ok, cur.kcol, a.k_plane, a.nvalid, a.center, a.fft.tw);
+    zproduct<L>(v, w, t, a.conj != 0, a.scale);
+    sfft_run<L, true, +1>(v, t, lds, addr, a.fft.stw);
+    cur.store(a, t, v);
+#pragma unroll
+    for (int i = 0; i < ZT::MAXE; ++i) v[i] = nv[i];
+  }
 }
 
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-template <int L>
-void sk_launch_xfwd(const XPassArgs& a, long npairs, hipStream_t st) {
-  constexpr int PAIRS = x_pairs_static<L>();
-  const unsigned grid = (unsigned)((npairs + PAIRS - 1) / PAIRS);
-  hipLaunchKernelGGL(sk_xfwd<L>, dim3(grid), dim3(PAIRS * PlanOf<L, false>::T), 0, st, a);
+template <int L, int PAIRS>
+bool sk_launch_xfwd(const XPass2& a, hipStream_t st) {
+  constexpr int P = fit_pairs<L, PAIRS>();
+  XPass2 b = a;
+  b.m_blocks = ((a.m.nrows + 1) / 2 + P - 1) / P;
+  b.k_blocks = ((a.k.nrows + 1) / 2 + P - 1) / P;
+  hipLaunchKernelGGL((sk_xfwd<L, P>), dim3((unsigned)(b.m_blocks + b.k_blocks)),
+                     dim3(P * threads_per_transform<L>()), 0, st, b);
+  return true;
 }
 
-template <int L>
-void sk_launch_xinv(const XPassArgs& a, long npairs, hipStream_t st) {
-  constexpr int PAIRS = x_pairs_static<L>();
-  const unsigned grid = (unsigned)((npairs + PAIRS - 1) / PAIRS);
-  hipLaunchKernelGGL(sk_xinv<L>, dim3(grid), dim3(PAIRS * PlanOf<L, false>::T), 0, st, a);
+template <int L, int PAIRS>
+bool sk_launch_xinv(const XPassArgs& a, hipStream_t st) {
+  constexpr int P = fit_pairs<L, PAIRS>();
+  const long blocks = ((a.nrows + 1) / 2 + P - 1) / P;
+  hipLaunchKernelGGL((sk_xinv<L, P>), dim3((unsigned)blocks), dim3(P * threads_per_transform<L>()),
+                     0, st, a);
+  return true;
 }
 
-template <int L>
-void sk_launch_col(const ColArgs& a, int dir, long nouter, hipStream_t st) {
-  constexpr int TX = col_tile_static<L>();
-  const dim3 grid((a.nbins + TX - 1) / TX, (unsigned)nouter);
-  const dim3 block(TX * PlanOf<L, false>::T);
+template <int L, int WANT>
+bool sk_launch_col(const Col2& a, int dir, hipStream_t st) {
+  constexpr int TX = fit_tile<L, WANT>();
+  Col2 b = a;
+  b.ntx = (a.m.nbins + TX - 1) / TX;
+  b.m_tiles = (long)b.ntx * a.m_outer;
+  b.ntiles = b.m_tiles + (long)b.ntx * a.k_outer;
+  const dim3 block(TX * threads_per_transform<L>());
   if (dir < 0)
-    hipLaunchKernelGGL((sk_col<L, -1>), grid, block, 0, st, a);
+    hipLaunchKernelGGL((sk_col<L, TX, -1>), dim3((unsigned)b.ntiles), block, 0, st, b);
   else
-    hipLaunchKernelGGL((sk_col<L, +1>), grid, block, 0, st, a);
+    hipLaunchKernelGGL((sk_col<L, TX, +1>), dim3((unsigned)b.ntiles), block, 0, st, b);
+  return true;
 }
 
+template <int L, int WANT>
+bool sk_launch_col_pipe(const Col2& a, int dir, hipStream_t st) {
+  constexpr int TX = fit_tile<L, WANT>();
+  constexpr int threads = TX * threads_per_transform<L>();
+  Col2 b = a;
+  b.ntx = (a.m.nbins + TX - 1) / TX;
+  b.m_tiles = (long)b.ntx * a.m_outer;
+  b.ntiles = b.m_tiles + (long)b.ntx * a.k_outer;
+  const void* fn = dir < 0 ? (const void*)sk_col_pipe<L, TX, -1> : (const void*)sk_col_pipe<L, TX, +1>;
+  long grid = persistent_blocks(fn, threads);
+  if (grid > b.ntiles) grid = b.ntiles;
+  if (grid < 1) grid = 1;
+  if (dir < 0)
+    hipLaunchKernelGGL((sk_col_pipe<L, TX, -1>), dim3((unsigned)grid), dim3(threads), 0, st, b);
+  else
+    hipLaunchKernelGGL((sk_col_pipe<L, TX, +1>), dim3((unsigned)grid), dim3(threads), 0, st, b);
+  return true;
+}
+
+template <int L, int WANT>
+bool sk_launch_zfused_fft(const ZArgs& a, long ny, hipStream_t st) {
+  constexpr int TX = fit_tile<L, WANT>();
+  const int ntx = (a.nbins + TX - 1) / TX;
+  hipLaunchKernelGGL((sk_zfused_fft<L, TX>), dim3((unsigned)(ntx * ny)),
+                     dim3(TX * threads_per_transform<L>()), 0, st, a, ntx);
+  return true;
+}
+
+// direct kernel DFT costs kz complex MACs per output; use it while that is
+// cheaper than a transform of the mostly-zero kernel column
 template <int L>
-void sk_launch_zfused(const ZArgs& a, long ny, hipStream_t st) {
-  constexpr int TX = col_tile_static<L>();
-  const dim3 grid((a.nbins + TX - 1) / TX, (unsigned)ny);
-  hipLaunchKernelGGL(sk_zfused<L>, grid, dim3(TX * PlanOf<L, false>::T), 0, st, a);
+constexpr int dft_kz_limit() {
+  return L / 4 > 64 ? 64 : L / 4;
+}
+
+template <int L, int WANT>
+bool sk_launch_zfused_dft(const ZArgs& a, long ny, hipStream_t st) {
+  if (a.nvalid > dft_kz_limit<L>()) return false;
+  constexpr int TX = fit_tile<L, WANT>();
+  const int ntx = (a.nbins + TX - 1) / TX;
+  hipLaunchKernelGGL((sk_zfused_dft<L, TX>), dim3((unsigned)(ntx * ny)),
+                     dim3(TX * threads_per_transform<L>()), 0, st, a, ntx);
+  return true;
+}
+
+template <int L, int WANT>
+bool sk_launch_zfused_dft_pipe(const ZArgs& a, long ny, hipStream_t st) {
+  if (a.nvalid > dft_kz_limit<L>()) return false;
+  constexpr int TX = fit_tile<L, WANT>();
+  constexpr int threads = TX * threads_per_transform<L>();
+  const int ntx = (a.nbins + TX - 1) / TX;
+  const long ntiles = (long)ntx * ny;
+  long grid = persistent_blocks((const void*)sk_zfused_dft_pipe<L, TX>, threads);
+  if (grid > ntiles) grid = ntiles;
+  if (grid < 1) grid = 1;
+  hipLaunchKernelGGL((sk_zfused_dft_pipe<L, TX>), dim3((unsigned)grid), dim3(threads), 0, st, a,
+                     ntx, ntiles);
+  return true;
 }
 
 template <int L>
 StaticEntry make_static_entry() {
   StaticEntry e;
-  e.xfwd = &sk_launch_xfwd<L>;
-  e.xinv = &sk_launch_xinv<L>;
-  e.col = &sk_launch_col<L>;
-  e.zfused = &sk_launch_zfused<L>;
+  e.xfwd = {{"p4", &sk_launch_xfwd<L, 4>}, {"p2", &sk_launch_xfwd<L, 2>}};
+  e.xinv = {{"p4", &sk_launch_xinv<L, 4>}, {"p2", &sk_launch_xinv<L, 2>}};
+  e.col = {{"t16", &sk_launch_col<L, 16>},
+           {"t16-pipe", &sk_launch_col_pipe<L, 16>},
+           {"t8-pipe", &sk_launch_col_pipe<L, 8>}};
+  e.zfused = {{"t16-fft", &sk_launch_zfused_fft<L, 16>},
+              {"t16-dft", &sk_launch_zfused_dft<L, 16>},
+              {"t16-dft-pipe", &sk_launch_zfused_dft_pipe<L, 16>},
+              {"t8-dft-pipe", &sk_launch_zfused_dft_pipe<L, 8>}};
   return e;
 }
 

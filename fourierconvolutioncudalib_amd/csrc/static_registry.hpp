@@ -1,19 +1,62 @@
-// static_registry.hpp -- lengths that have compile-time kernels, and their
-// launchers.  Filled at load time by the instantiation units.
+// static_registry.hpp -- lengths that have compile-time kernels, their
+// variants (tile shape, pipelining, kernel-spectrum method) and launchers.
+// Filled at load time by the instantiation units; the autotuner picks the
+// variant per (pass, length).
 #pragma once
+
+#include <vector>
 
 #include "kernels.hpp"
 
 namespace fcamd {
 
+enum StaticPass : int { kSxFwd = 0, kSxInv = 1, kSCol = 2, kSZ = 3, kSNumPasses = 4 };
+
+// Every launcher returns false when its variant does not apply to the
+// arguments (the dispatcher then uses variant 0).
+using XFwdFn = bool (*)(const XPass2&, hipStream_t);
+using XInvFn = bool (*)(const XPassArgs&, hipStream_t);
+using ColFn = bool (*)(const Col2&, int dir, hipStream_t);
+using ZFn = bool (*)(const ZArgs&, long ny, hipStream_t);
+
+template <class F>
+struct Variant {
+  const char* name;
+  F fn;
+};
+
 struct StaticEntry {
-  void (*xfwd)(const XPassArgs&, long npairs, hipStream_t) = nullptr;
-  void (*xinv)(const XPassArgs&, long npairs, hipStream_t) = nullptr;
-  void (*col)(const ColArgs&, int dir, long nouter, hipStream_t) = nullptr;
-  void (*zfused)(const ZArgs&, long ny, hipStream_t) = nullptr;
+  std::vector<Variant<XFwdFn>> xfwd;
+  std::vector<Variant<XInvFn>> xinv;
+  std::vector<Variant<ColFn>> col;
+  std::vector<Variant<ZFn>> zfused;
+  int selected[kSNumPasses] = {0, 0, 0, 0};
+  int count(int pass) const {
+    switch (pass) {
+      case kSxFwd: return (int)xfwd.size();
+      case kSxInv: return (int)xinv.size();
+      case kSCol: return (int)col.size();
+      case kSZ: return (int)zfused.size();
+      default: return 0;
+    }
+  }
+  const char* name(int pass, int i) const {
+    if (i < 0 || i >= count(pass)) return nullptr;
+    switch (pass) {
+      case kSxFwd: return xfwd[i].name;
+      case kSxInv: return xinv[i].name;
+      case kSCol: return col[i].name;
+      default: return zfused[i].name;
+    }
+  }
 };
 
 void static_register(int L, const StaticEntry& e);
-const StaticEntry* static_lookup(int L);
+// Entry for length L (nullptr when none or when compile-time kernels are
+// switched off).  The pointer stays valid for the life of the process.
+StaticEntry* static_lookup(int L);
+
+// Persistent-grid size for a kernel: resident workgroups per CU x CUs.
+int persistent_blocks(const void* kernel, int threads);
 
 }  // namespace fcamd

@@ -44,6 +44,19 @@ FCAMD_API int fcamd_convolve3d_device(float* d_out, const float* d_image, const 
                                       const float* d_kernel, const int* kernelDim, int op,
                                       void* d_workspace, long long workspace_bytes, void* stream);
 
+/* Adjoint with respect to the kernel (no counterpart in the reference; the
+ * gradient half of the differentiable form).  For out = conv(im, k) and an
+ * upstream gradient g (image-sized), writes
+ *   dk[j] = sum_i g[i] * im[(i - (j - floor(k/2))) mod n]
+ * for every kernel tap j (kernelDim-sized, device memory).  The gradient
+ * with respect to the image is fcamd_convolve3d_device(..., FCAMD_CORRELATE,
+ * ...) applied to g.  Workspace as for fcamd_convolve3d_device. */
+FCAMD_API long long fcamd_grad_kernel_workspace_bytes(const int* imDim, const int* kernelDim);
+FCAMD_API int fcamd_conv_grad_kernel_device(float* d_dkernel, const float* d_grad_out,
+                                            const float* d_image, const int* imDim,
+                                            const int* kernelDim, void* d_workspace,
+                                            long long workspace_bytes, void* stream);
+
 /* Per-pass HIP-event timing of the convolution passes (diagnostics and the
  * benchmark's roofline figures).  While enabled every pass launch is
  * bracketed by two events on its stream.  fcamd_pass_times waits for them,
@@ -78,6 +91,15 @@ FCAMD_API int fcamd_has_static_kernel(int pass, int length);
 /* Lists the lengths with compile-time kernels (up to `cap`); returns how many
  * there are. */
 FCAMD_API int fcamd_static_lengths(int* lengths, int cap);
+
+/* Variants of the compile-time kernels (pass: 0 x forward, 1 x inverse,
+ * 2 y column, 3 fused z).  Names describe the tile shape and method; the
+ * selection is per (pass, length) and process-wide; fcamd_autotune sets it
+ * from timings.  select returns 0 or an error code. */
+FCAMD_API int fcamd_static_variant_count(int pass, int length);
+FCAMD_API const char* fcamd_static_variant_name(int pass, int length, int index);
+FCAMD_API int fcamd_select_static_variant(int pass, int length, int index);
+FCAMD_API int fcamd_selected_static_variant(int pass, int length);
 
 /* enable = 0 routes every pass through the runtime-length kernels (for
  * cross-checks and comparisons); 1 restores the default.  Process-wide. */

@@ -75,3 +75,28 @@ def test_correlate_static(fc, oracle):
     k = synth.random_psf((9, 11, 7), 3)
     out = fc.convolve(img, k, op=fc.CORRELATE)
     assert oracle.rel_l2(out, oracle.numpy_fft_conv(img, k, conj_kernel=True)) < TOL
+
+
+@pytest.mark.parametrize("L", [512, 256, 600, 105, 64, 1024])
+def test_every_variant(fc, oracle, L):
+    """Each compile-time variant of each pass, selected in turn, against the
+    FP64 oracle (small and large kz for the fused z pass)."""
+    lib = fc.lib()
+    cases = {0: [(6, 10, L)], 1: [(6, 10, L)], 2: [(6, L, 24)],
+             3: [(L, 10, 24), (L, 6, 18)]}
+    try:
+        for p, shapes in cases.items():
+            n = lib.fcamd_static_variant_count(p, L)
+            assert n >= 1
+            for v in range(n):
+                assert lib.fcamd_select_static_variant(p, L, v) == 0
+                for si, shape in enumerate(shapes):
+                    if si == 0:
+                        kshape = tuple(min(s, k) for s, k in zip(shape, (5, 7, 3)))
+                    else:  # kz beyond the direct-DFT limit -> fallback variant
+                        kshape = (min(L, 61 if L >= 128 else L - 1), 5, 3)
+                    _check(fc, oracle, shape, kshape, 10 * p + v)
+            lib.fcamd_select_static_variant(p, L, 0)
+    finally:
+        for p in cases:
+            lib.fcamd_select_static_variant(p, L, 0)
