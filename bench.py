@@ -90,6 +90,56 @@ def cpu_baseline(img, psf, planes_per_thread=2):
     }
 
 
+def batched_comparison(fc, torch, reps=5):
+    """64 uneven menu blocks: batched plan (eager and hipGraph replay) vs the
+    plain per-block loop of fcamd_convolve3d_device calls on one stream."""
+    from fourierconvolutioncudalib_amd import synth
+    from fourierconvolutioncudalib_amd.batched import BatchedConvolution
+    spec = synth.uneven_block_shapes(3003, 64)
+    g = torch.Generator(device="cuda").manual_seed(3003)
+    imgs = [torch.rand(s, device="cuda", generator=g) for s, _, _ in spec]
+    kers = [torch.from_numpy(synth.random_psf(k, sd)).cuda() for _, k, sd in spec]
+    outs = [torch.empty_like(t) for t in imgs]
+    ws = torch.empty(max(fc.workspace_bytes(s, k) for s, k, _ in spec), dtype=torch.uint8,
+                     device="cuda")
+    voxels = sum(int(t.numel()) for t in imgs)
+    stream = torch.cuda.current_stream()
+
+    def plain():
+        for a, k, o in zip(imgs, kers, outs):
+            fc.convolve3d(a, k, out=o, workspace=ws, stream=stream)
+
+    plan = BatchedConvolution(imgs, kers, outs=[torch.empty_like(t) for t in imgs])
+
+    def timed(fn):
+        for _ in range(2):
+            fn()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps, (time.perf_counter() - t0) * 1e3 / reps
+
+    t_plain, w_plain = timed(plain)
+    t_eager, w_eager = timed(lambda: plan.run(graph=False))
+    t_graph, w_graph = timed(lambda: plan.run(graph=True))
+    plan.close()
+    return {
+        "workload": "64 uneven blocks, extents from {64..256} 7-smooth menu, odd 9-41 kernels",
+        "voxels": voxels,
+        "plain_loop_ms": t_plain, "batched_eager_ms": t_eager, "batched_graph_ms": t_graph,
+        "plain_loop_wall_ms": w_plain, "batched_graph_wall_ms": w_graph,
+        "batched_voxels_per_s": voxels / (t_graph * 1e-3),
+        "speedup_graph_vs_plain": t_plain / t_graph,
+        "speedup_eager_vs_plain": t_plain / t_eager,
+    }
+
+
 def load_traffic(workload):
     path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
     if not os.path.exists(path):
@@ -104,6 +154,7 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -234,6 +285,8 @@ def main():
         "wall_s_timed_region": wall,
         "host_api_ms_pcie_inclusive": min(t_host) * 1e3,
     }
+    if not args.no_batched:
+        result["batched"] = batched_comparison(fc, torch)
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(img_np, psf_np)
     else:

@@ -52,6 +52,11 @@ SIGNATURES = {
     "fcamd_static_variant_name": (ctypes.c_char_p, [_i, _i, _i]),
     "fcamd_select_static_variant": (_i, [_i, _i, _i]),
     "fcamd_selected_static_variant": (_i, [_i, _i]),
+    "fcamd_batch_workspace_bytes": (_ll, [_i, _ip, _ip]),
+    "fcamd_batch_create": (_vp, [_i, _ip, _ip, _vp, _vp, _vp, _i, _vp, _ll]),
+    "fcamd_batch_execute": (_i, [_vp, _vp]),
+    "fcamd_batch_graph_launch": (_i, [_vp, _vp]),
+    "fcamd_batch_destroy": (None, [_vp]),
     "fcamd_pass_timing": (None, [_i]),
     "fcamd_pass_times": (_i, [ctypes.POINTER(ctypes.c_double), _ip, _i]),
     "fcamd_pass_name": (ctypes.c_char_p, [_i]),

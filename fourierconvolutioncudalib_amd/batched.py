@@ -1,0 +1,66 @@
+"""Batched convolution of many uneven blocks (torch CUDA tensors).
+
+Each block has its own image shape and kernel.  The library groups the
+blocks of every pass by transform length, launches one compile-time kernel
+per length (length groups on forked streams) and can capture the whole pass
+sequence in a hipGraph that is replayed on later calls.  Buffers are bound
+when the plan is created.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import torch
+
+from . import CONVOLVE, _stream_handle
+from ._lib import FCError, check, ints, lib
+
+
+class BatchedConvolution:
+    """Plan over lists of image / kernel tensors ([nz, ny, nx] / [kz, ky, kx],
+    contiguous float32 on one device).  ``outs`` defaults to new tensors;
+    pass the images themselves for in-place results."""
+
+    def __init__(self, images, kernels, outs=None, op=CONVOLVE):
+        if len(images) != len(kernels) or not images:
+            raise ValueError("need one kernel per image")
+        self.images = [t.contiguous() for t in images]
+        self.kernels = [t.contiguous() for t in kernels]
+        for t in self.images + self.kernels:
+            if t.dtype != torch.float32 or not t.is_cuda or t.dim() != 3:
+                raise TypeError("3D float32 CUDA tensors required")
+        self.outs = outs if outs is not None else [torch.empty_like(t) for t in self.images]
+        n = len(self.images)
+        self.device = self.images[0].device
+        imdims = ints([d for t in self.images for d in t.shape[::-1]])
+        kdims = ints([d for t in self.kernels for d in t.shape[::-1]])
+        ptrs = lambda ts: (ctypes.c_void_p * n)(*[t.data_ptr() for t in ts])  # noqa: E731
+        with torch.cuda.device(self.device):
+            self._plan = lib().fcamd_batch_create(n, imdims, kdims, ptrs(self.outs),
+                                                  ptrs(self.images), ptrs(self.kernels), int(op),
+                                                  None, 0)
+        if not self._plan:
+            check(lib().fcamd_last_error() or 6)
+            raise FCError(6, "batch plan creation failed")
+
+    def run(self, stream=None, graph=True):
+        """Enqueue all blocks on ``stream`` (default: torch's current stream);
+        ``graph`` replays the captured hipGraph."""
+        if self._plan is None:
+            raise RuntimeError("plan closed")
+        with torch.cuda.device(self.device):
+            h = _stream_handle(stream)
+            fn = lib().fcamd_batch_graph_launch if graph else lib().fcamd_batch_execute
+            check(fn(self._plan, h))
+        return self.outs
+
+    def close(self):
+        if self._plan is not None:
+            lib().fcamd_batch_destroy(self._plan)
+            self._plan = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,476 @@
+// batch.cpp -- many convolutions of uneven shapes as one pass sequence.
+//
+// Every block (image, kernel, output; its own shape) goes through the same
+// five passes as a single convolution.  For each pass the blocks are
+// grouped by the transform length of that pass's axis; each group is one
+// launch of the compile-time kernel for that length over a device task
+// table {argument record, tile}, so a pass costs one launch per distinct
+// length instead of one per block.  Groups of one pass are independent and
+// run on forked streams; the pass boundary joins them.  The whole sequence
+// can be captured once into a hipGraph and replayed.
+//
+// Blocks whose length has no compile-time kernel fall back to the
+// runtime-length kernels, one launch per block.
+#include <algorithm>
+#include <map>
+#include <memory>
+#include <vector>
+
+#include "../../include/fourierconvolution_amd.h"
+#include "conv_driver.hpp"
+#include "static_registry.hpp"
+
+namespace fcamd {
+
+namespace {
+
+size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+
+struct Block {
+  ConvShape s;
+  float* out;
+  const float* img;
+  const float* ker;
+  float2* S;
+  float2* Kx;
+  float2* Kxy;
+};
+
+template <class A>
+struct Group {
+  int L = 0;
+  StaticEntry* e = nullptr;
+  std::vector<A> args;
+  std::vector<int2> tasks;
+  A* d_args = nullptr;
+  int2* d_tasks = nullptr;
+};
+
+template <class A>
+void upload(Group<A>& g) {
+  FC_HIP_CHECK(hipMalloc(&g.d_args, sizeof(A) * std::max<size_t>(1, g.args.size())));
+  FC_HIP_CHECK(hipMalloc(&g.d_tasks, sizeof(int2) * std::max<size_t>(1, g.tasks.size())));
+  FC_HIP_CHECK(hipMemcpy(g.d_args, g.args.data(), sizeof(A) * g.args.size(), hipMemcpyHostToDevice));
+  FC_HIP_CHECK(hipMemcpy(g.d_tasks, g.tasks.data(), sizeof(int2) * g.tasks.size(),
+                         hipMemcpyHostToDevice));
+}
+
+template <class A>
+void release(Group<A>& g) {
+  if (g.d_args) (void)hipFree(g.d_args);
+  if (g.d_tasks) (void)hipFree(g.d_tasks);
+  g.d_args = nullptr;
+  g.d_tasks = nullptr;
+}
+
+void add_tasks(std::vector<int2>& tasks, int arg, long count) {
+  FC_REQUIRE(count < (1L << 31), kUnsupportedSize, "batch block too large for the task table");
+  for (long i = 0; i < count; ++i) tasks.push_back(make_int2(arg, (int)i));
+}
+
+}  // namespace
+
+class BatchPlan {
+ public:
+  BatchPlan(int n, const int* imDims, const int* kernelDims, float* const* outs,
+            const float* const* imgs, const float* const* kers, int op, void* ws, size_t ws_bytes);
+  ~BatchPlan();
+  void enqueue(hipStream_t st);
+  void graph_launch(hipStream_t st);
+  size_t workspace() const { return ws_need_; }
+
+ private:
+  template <class F>
+  void fork(hipStream_t st, size_t ngroups, F launch_group);
+  void build_groups();
+
+  int dev_ = 0;
+  bool conj_ = false;
+  std::vector<Block> blocks_;
+  size_t ws_need_ = 0;
+  void* ws_ = nullptr;
+  bool own_ws_ = false;
+  // per pass
+  std::vector<Group<XPassArgs>> xf_, xi_;
+  std::vector<Group<ColArgs>> yf_, yi_;
+  std::vector<Group<ZArgs>> z_;
+  std::vector<int> generic_x_, generic_y_, generic_z_;  // block indices
+  std::vector<hipStream_t> side_;
+  std::vector<hipEvent_t> ev_;
+  hipGraphExec_t exec_ = nullptr;
+  hipGraph_t graph_ = nullptr;
+  hipStream_t cap_ = nullptr;
+};
+
+BatchPlan::BatchPlan(int n, const int* imDims, const int* kernelDims, float* const* outs,
+                     const float* const* imgs, const float* const* kers, int op, void* ws,
+                     size_t ws_bytes) {
+  FC_REQUIRE(n >= 1, kInvalidArgument, "empty batch");
+  FC_REQUIRE(imDims && kernelDims && outs && imgs && kers, kInvalidArgument, "null array");
+  FC_REQUIRE(op == FCAMD_CONVOLVE || op == FCAMD_CORRELATE, kInvalidArgument, "unknown op");
+  conj_ = op == FCAMD_CORRELATE;
+  FC_HIP_CHECK(hipGetDevice(&dev_));
+  size_t off = 0;
+  std::vector<size_t> offs;
+  for (int b = 0; b < n; ++b) {
+    Block bl{};
+    bl.s.nx = imDims[3 * b];
+    bl.s.ny = imDims[3 * b + 1];
+    bl.s.nz = imDims[3 * b + 2];
+    bl.s.kx = kernelDims[3 * b];
+    bl.s.ky = kernelDims[3 * b + 1];
+    bl.s.kz = kernelDims[3 * b + 2];
+    bl.s.cx = bl.s.kx / 2;
+    bl.s.cy = bl.s.ky / 2;
+    bl.s.cz = bl.s.kz / 2;
+    validate_shape(bl.s);
+    FC_REQUIRE(outs[b] && imgs[b] && kers[b], kInvalidArgument, "null block pointer");
+    bl.out = outs[b];
+    bl.img = imgs[b];
+    bl.ker = kers[b];
+    offs.push_back(off);
+    off += align256(workspace_layout(bl.s).total());
+    blocks_.push_back(bl);
+  }
+  ws_need_ = off;
+  if (ws) {
+    FC_REQUIRE(ws_bytes >= ws_need_, kInvalidArgument, "batch workspace too small");
+    ws_ = ws;
+  } else {
+    FC_HIP_CHECK(hipMalloc(&ws_, ws_need_));
+    own_ws_ = true;
+  }
+  for (int b = 0; b < n; ++b) {
+    const WorkspaceLayout wl = workspace_layout(blocks_[b].s);
+    char* base = static_cast<char*>(ws_) + offs[b];
+    blocks_[b].S = reinterpret_cast<float2*>(base);
+    blocks_[b].Kx = reinterpret_cast<float2*>(base + wl.spectrum);
+    blocks_[b].Kxy = reinterpret_cast<float2*>(base + wl.spectrum + wl.krows);
+  }
+  build_groups();
+}
+
+void BatchPlan::build_groups() {
+  driver_init();
+  std::map<int, std::vector<int>> by_x, by_y, by_z;
+  // create every twiddle table now: nothing may allocate or copy while a
+  // graph is being captured
+  for (const Block& bl : blocks_) {
+    (void)fft_desc(bl.s.nx);
+    (void)fft_desc(bl.s.ny);
+    (void)fft_desc(bl.s.nz);
+  }
+  for (int b = 0; b < (int)blocks_.size(); ++b) {
+    by_x[blocks_[b].s.nx].push_back(b);
+    by_y[blocks_[b].s.ny].push_back(b);
+    by_z[blocks_[b].s.nz].push_back(b);
+  }
+  // x forward / inverse
+  for (auto& kv : by_x) {
+    StaticEntry* e = static_lookup(kv.first);
+    if (!e || !e->batch.xfwd) {
+      generic_x_.insert(generic_x_.end(), kv.second.begin(), kv.second.end());
+      continue;
+    }
+    const FftDesc fx = fft_desc(kv.first);
+    const int P = e->batch.pairs;
+    Group<XPassArgs> gf, gi;
+    gf.L = gi.L = kv.first;
+    gf.e = gi.e = e;
+    for (int b : kv.second) {
+      const Block& bl = blocks_[b];
+      const long nxp = padded_bins(bl.s.nx);
+      XPassArgs m{};
+      m.fft = fx;
+      m.nrows = (long)bl.s.ny * bl.s.nz;
+      m.nbins = half_bins(bl.s.nx);
+      m.src = bl.img;
+      m.dst = bl.S;
+      m.src_row = bl.s.nx;
+      m.dst_row = nxp;
+      m.klen = bl.s.nx;
+      m.kcenter = 0;
+      XPassArgs k = m;
+      k.nrows = (long)bl.s.ky * bl.s.kz;
+      k.src = bl.ker;
+      k.dst = bl.Kx;
+      k.src_row = bl.s.kx;
+      k.klen = bl.s.kx;
+      k.kcenter = bl.s.cx;
+      k.kshift = 1;
+      // kernel rows first (they gate the kernel planes of the next pass)
+      gf.args.push_back(k);
+      add_tasks(gf.tasks, (int)gf.args.size() - 1, ((k.nrows + 1) / 2 + P - 1) / P);
+      gf.args.push_back(m);
+      add_tasks(gf.tasks, (int)gf.args.size() - 1, ((m.nrows + 1) / 2 + P - 1) / P);
+      XPassArgs iv{};
+      iv.fft = fx;
+      iv.nrows = m.nrows;
+      iv.nbins = m.nbins;
+      iv.csrc = bl.S;
+      iv.rdst = bl.out;
+      iv.src_row = nxp;
+      iv.dst_row = bl.s.nx;
+      gi.args.push_back(iv);
+      add_tasks(gi.tasks, (int)gi.args.size() - 1, ((iv.nrows + 1) / 2 + P - 1) / P);
+    }
+    upload(gf);
+    upload(gi);
+    xf_.push_back(gf);
+    xi_.push_back(gi);
+  }
+  // y forward (+ kernel planes) / inverse
+  for (auto& kv : by_y) {
+    StaticEntry* e = static_lookup(kv.first);
+    if (!e || !e->batch.col) {
+      generic_y_.insert(generic_y_.end(), kv.second.begin(), kv.second.end());
+      continue;
+    }
+    const FftDesc fy = fft_desc(kv.first);
+    const int TX = e->batch.tx;
+    Group<ColArgs> gf, gi;
+    gf.L = gi.L = kv.first;
+    gf.e = gi.e = e;
+    for (int b : kv.second) {
+      const Block& bl = blocks_[b];
+      const long nxp = padded_bins(bl.s.nx);
+      const int nb = half_bins(bl.s.nx);
+      const long ntx = (nb + TX - 1) / TX;
+      ColArgs m{};
+      m.fft = fy;
+      m.nbins = nb;
+      m.src = bl.S;
+      m.dst = bl.S;
+      m.src_outer = m.dst_outer = (long)bl.s.ny * nxp;
+      m.src_elem = m.dst_elem = nxp;
+      m.nvalid = bl.s.ny;
+      m.center = 0;
+      ColArgs k = m;
+      k.src = bl.Kx;
+      k.dst = bl.Kxy;
+      k.src_outer = (long)bl.s.ky * nxp;
+      k.nvalid = bl.s.ky;
+      k.center = bl.s.cy;
+      gf.args.push_back(k);
+      add_tasks(gf.tasks, (int)gf.args.size() - 1, ntx * bl.s.kz);
+      gf.args.push_back(m);
+      add_tasks(gf.tasks, (int)gf.args.size() - 1, ntx * bl.s.nz);
+      gi.args.push_back(m);
+      add_tasks(gi.tasks, (int)gi.args.size() - 1, ntx * bl.s.nz);
+    }
+    upload(gf);
+    upload(gi);
+    yf_.push_back(gf);
+    yi_.push_back(gi);
+  }
+  // fused z
+  for (auto& kv : by_z) {
+    StaticEntry* e = static_lookup(kv.first);
+    if (!e || !e->batch.zfused) {
+      generic_z_.insert(generic_z_.end(), kv.second.begin(), kv.second.end());
+      continue;
+    }
+    const FftDesc fz = fft_desc(kv.first);
+    const int TX = e->batch.tx;
+    Group<ZArgs> g;
+    g.L = kv.first;
+    g.e = e;
+    for (int b : kv.second) {
+      const Block& bl = blocks_[b];
+      const long nxp = padded_bins(bl.s.nx);
+      ZArgs a{};
+      a.fft = fz;
+      a.nbins = half_bins(bl.s.nx);
+      a.s = bl.S;
+      a.s_row = nxp;
+      a.s_plane = (long)bl.s.ny * nxp;
+      a.k = bl.Kxy;
+      a.k_row = nxp;
+      a.k_plane = (long)bl.s.ny * nxp;
+      a.nvalid = bl.s.kz;
+      a.center = bl.s.cz;
+      a.conj = conj_ ? 1 : 0;
+      a.scale = (float)(1.0 / (double)bl.s.voxels());
+      g.args.push_back(a);
+      add_tasks(g.tasks, (int)g.args.size() - 1, (long)((a.nbins + TX - 1) / TX) * bl.s.ny);
+    }
+    upload(g);
+    z_.push_back(g);
+  }
+  size_t nside = std::max({xf_.size(), yf_.size(), z_.size(), (size_t)1});
+  for (size_t i = 0; i < nside; ++i) {
+    hipStream_t s = nullptr;
+    FC_HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    side_.push_back(s);
+  }
+  for (size_t i = 0; i < nside + 1; ++i) {
+    hipEvent_t e = nullptr;
+    FC_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    ev_.push_back(e);
+  }
+}
+
+BatchPlan::~BatchPlan() {
+  if (exec_) (void)hipGraphExecDestroy(exec_);
+  if (graph_) (void)hipGraphDestroy(graph_);
+  for (auto& g : xf_) release(g);
+  for (auto& g : xi_) release(g);
+  for (auto& g : yf_) release(g);
+  for (auto& g : yi_) release(g);
+  for (auto& g : z_) release(g);
+  for (auto s : side_) (void)hipStreamDestroy(s);
+  if (cap_) (void)hipStreamDestroy(cap_);
+  for (auto e : ev_) (void)hipEventDestroy(e);
+  if (own_ws_ && ws_) (void)hipFree(ws_);
+}
+
+// Runs launch_group(i, stream) for every group i, group 0 on `st` and the
+// others on side streams forked from and joined back into `st`.
+template <class F>
+void BatchPlan::fork(hipStream_t st, size_t ngroups, F launch_group) {
+  if (ngroups == 0) return;
+  if (ngroups > 1) {
+    FC_HIP_CHECK(hipEventRecord(ev_[0], st));
+    for (size_t i = 1; i < ngroups; ++i) FC_HIP_CHECK(hipStreamWaitEvent(side_[i - 1], ev_[0], 0));
+  }
+  for (size_t i = 0; i < ngroups; ++i) launch_group(i, i == 0 ? st : side_[i - 1]);
+  for (size_t i = 1; i < ngroups; ++i) {
+    FC_HIP_CHECK(hipEventRecord(ev_[i], side_[i - 1]));
+    FC_HIP_CHECK(hipStreamWaitEvent(st, ev_[i], 0));
+  }
+}
+
+void BatchPlan::enqueue(hipStream_t st) {
+  // x forward (image and kernel rows)
+  fork(st, xf_.size(), [&](size_t i, hipStream_t s) {
+    auto& g = xf_[i];
+    g.e->batch.xfwd(g.d_args, g.d_tasks, (int)g.tasks.size(), s);
+  });
+  for (int b : generic_x_) {
+    const Block& bl = blocks_[b];
+    pass_x_forward(bl.s, bl.img, bl.S, bl.ker, bl.Kx, st);
+  }
+  // y forward (image and kernel planes); blocks already done above skip
+  fork(st, yf_.size(), [&](size_t i, hipStream_t s) {
+    auto& g = yf_[i];
+    g.e->batch.col(g.d_args, g.d_tasks, (int)g.tasks.size(), -1, s);
+  });
+  for (int b : generic_y_) {
+    const Block& bl = blocks_[b];
+    pass_y_forward(bl.s, bl.S, bl.Kx, bl.Kxy, st);
+  }
+  fork(st, z_.size(), [&](size_t i, hipStream_t s) {
+    auto& g = z_[i];
+    g.e->batch.zfused(g.d_args, g.d_tasks, (int)g.tasks.size(), s);
+  });
+  for (int b : generic_z_) {
+    const Block& bl = blocks_[b];
+    z_pass(bl.s, bl.S, bl.Kxy, false, conj_, st);
+  }
+  fork(st, yi_.size(), [&](size_t i, hipStream_t s) {
+    auto& g = yi_[i];
+    g.e->batch.col(g.d_args, g.d_tasks, (int)g.tasks.size(), +1, s);
+  });
+  for (int b : generic_y_) pass_y_inverse(blocks_[b].s, blocks_[b].S, st);
+  fork(st, xi_.size(), [&](size_t i, hipStream_t s) {
+    auto& g = xi_[i];
+    g.e->batch.xinv(g.d_args, g.d_tasks, (int)g.tasks.size(), s);
+  });
+  for (int b : generic_x_) pass_x_inverse(blocks_[b].s, blocks_[b].S, blocks_[b].out, st);
+  FC_HIP_CHECK(hipGetLastError());
+}
+
+void BatchPlan::graph_launch(hipStream_t st) {
+  if (!exec_) {
+    // capture on a plan-owned stream (the caller's may be the null stream,
+    // which cannot be captured); the graph is then launched on `st`
+    if (!cap_) FC_HIP_CHECK(hipStreamCreateWithFlags(&cap_, hipStreamNonBlocking));
+    FC_HIP_CHECK(hipStreamBeginCapture(cap_, hipStreamCaptureModeThreadLocal));
+    try {
+      enqueue(cap_);
+    } catch (...) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(cap_, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    FC_HIP_CHECK(hipStreamEndCapture(cap_, &graph_));
+    FC_HIP_CHECK(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0));
+  }
+  FC_HIP_CHECK(hipGraphLaunch(exec_, st));
+}
+
+}  // namespace fcamd
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+using namespace fcamd;
+
+namespace {
+template <class F>
+int batch_guarded(F&& f) {
+  clear_error();
+  try {
+    f();
+    return kOk;
+  } catch (const Error& e) {
+    set_error(e.code, e.msg);
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    set_error(kOutOfMemory, "host allocation failed");
+    return kOutOfMemory;
+  } catch (...) {
+    set_error(kInternal, "unexpected internal error");
+    return kInternal;
+  }
+}
+}  // namespace
+
+extern "C" {
+
+long long fcamd_batch_workspace_bytes(int nblocks, const int* imDims, const int* kernelDims) {
+  long long r = -1;
+  (void)batch_guarded([&] {
+    FC_REQUIRE(nblocks >= 1 && imDims && kernelDims, kInvalidArgument, "bad batch description");
+    size_t tot = 0;
+    for (int b = 0; b < nblocks; ++b) {
+      ConvShape s{imDims[3 * b], imDims[3 * b + 1], imDims[3 * b + 2],
+                  kernelDims[3 * b], kernelDims[3 * b + 1], kernelDims[3 * b + 2],
+                  kernelDims[3 * b] / 2, kernelDims[3 * b + 1] / 2, kernelDims[3 * b + 2] / 2};
+      validate_shape(s);
+      tot += align256(workspace_layout(s).total());
+    }
+    r = (long long)tot;
+  });
+  return r;
+}
+
+void* fcamd_batch_create(int nblocks, const int* imDims, const int* kernelDims, float* const* d_outs,
+                         const float* const* d_images, const float* const* d_kernels, int op,
+                         void* d_workspace, long long workspace_bytes) {
+  BatchPlan* p = nullptr;
+  (void)batch_guarded([&] {
+    p = new BatchPlan(nblocks, imDims, kernelDims, d_outs, d_images, d_kernels, op, d_workspace,
+                      (size_t)(workspace_bytes < 0 ? 0 : workspace_bytes));
+  });
+  return p;
+}
+
+int fcamd_batch_execute(void* plan, void* stream) {
+  return batch_guarded([&] {
+    FC_REQUIRE(plan != nullptr, kInvalidArgument, "null batch plan");
+    static_cast<BatchPlan*>(plan)->enqueue(static_cast<hipStream_t>(stream));
+  });
+}
+
+int fcamd_batch_graph_launch(void* plan, void* stream) {
+  return batch_guarded([&] {
+    FC_REQUIRE(plan != nullptr, kInvalidArgument, "null batch plan");
+    static_cast<BatchPlan*>(plan)->graph_launch(static_cast<hipStream_t>(stream));
+  });
+}
+
+void fcamd_batch_destroy(void* plan) { delete static_cast<BatchPlan*>(plan); }
+
+}  // extern "C"

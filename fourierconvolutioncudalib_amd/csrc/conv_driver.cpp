@@ -257,14 +257,16 @@ void pass_timing_collect(double* ms, int* count) {
   }
 }
 
-namespace {
-
-void ensure_kernel_attributes() {
+void driver_init() {
   std::call_once(g_attr_once, [] {
     generic_kernels_allow_large_lds();
     static_kernels_allow_large_lds();
   });
 }
+
+namespace {
+
+void ensure_kernel_attributes() { driver_init(); }
 
 ColArgs spectrum_column_args(const ConvShape& s, const FftDesc& fy, float2* S) {
   const long nxp = padded_bins(s.nx);
@@ -283,12 +285,12 @@ ColArgs spectrum_column_args(const ConvShape& s, const FftDesc& fy, float2* S) {
 
 }  // namespace
 
-void forward_xy(const ConvShape& s, const float* src, float2* S, const float* kernel, float2* Kx,
-                float2* Kxy, hipStream_t st) {
+void pass_x_forward(const ConvShape& s, const float* src, float2* S, const float* kernel,
+                    float2* Kx, hipStream_t st) {
   ensure_kernel_attributes();
   const int nb = half_bins(s.nx);
   const long nxp = padded_bins(s.nx);
-  const FftDesc fx = fft_desc(s.nx), fy = fft_desc(s.ny);
+  const FftDesc fx = fft_desc(s.nx);
   const long nrows = (long)s.ny * s.nz;
   // x-R2C of the image rows (and of the kernel rows, in the same launch
   // when a compile-time kernel serves nx)
@@ -319,10 +321,18 @@ void forward_xy(const ConvShape& s, const float* src, float2* S, const float* ke
       }
     }
   }
+}
+
+void pass_y_forward(const ConvShape& s, float2* S, const float2* Kx, float2* Kxy, hipStream_t st) {
+  ensure_kernel_attributes();
+  const int nb = half_bins(s.nx);
+  const long nxp = padded_bins(s.nx);
+  const FftDesc fy = fft_desc(s.ny);
+  const bool kernel = Kx != nullptr;
   // y-FFT of the spectrum (and kernel rows -> kernel planes)
   {
     PassScope ps(kPassYForward, st);
-    if (!static_ycol(s, fy, S, -1, kernel ? Kx : nullptr, Kxy, st)) {
+    if (!static_ycol(s, fy, S, -1, Kx, Kxy, st)) {
       if (s.ny > 1) launch_gk_col(spectrum_column_args(s, fy, S), -1, s.nz, st);
       if (kernel) {
         PassScope pk(kPassKernelPlanes, st);
@@ -342,6 +352,12 @@ void forward_xy(const ConvShape& s, const float* src, float2* S, const float* ke
       }
     }
   }
+}
+
+void forward_xy(const ConvShape& s, const float* src, float2* S, const float* kernel, float2* Kx,
+                float2* Kxy, hipStream_t st) {
+  pass_x_forward(s, src, S, kernel, Kx, st);
+  pass_y_forward(s, S, kernel ? Kx : nullptr, Kxy, st);
 }
 
 void z_pass(const ConvShape& s, float2* S, const float2* operand, bool dense, bool conj,
@@ -366,14 +382,24 @@ void z_pass(const ConvShape& s, float2* S, const float2* operand, bool dense, bo
   if (!static_zfused(s, a, st)) launch_gk_zfused(a, s.ny, st);
 }
 
-void inverse_yx(const ConvShape& s, float2* S, float* dst, hipStream_t st) {
+void pass_y_inverse(const ConvShape& s, float2* S, hipStream_t st) {
   ensure_kernel_attributes();
-  const FftDesc fx = fft_desc(s.nx), fy = fft_desc(s.ny);
+  const FftDesc fy = fft_desc(s.ny);
   if (s.ny > 1) {
     PassScope ps(kPassYInverse, st);
     if (!static_ycol(s, fy, S, +1, nullptr, nullptr, st))
       launch_gk_col(spectrum_column_args(s, fy, S), +1, s.nz, st);
   }
+}
+
+void inverse_yx(const ConvShape& s, float2* S, float* dst, hipStream_t st) {
+  pass_y_inverse(s, S, st);
+  pass_x_inverse(s, S, dst, st);
+}
+
+void pass_x_inverse(const ConvShape& s, const float2* S, float* dst, hipStream_t st) {
+  ensure_kernel_attributes();
+  const FftDesc fx = fft_desc(s.nx);
   PassScope ps(kPassXInverse, st);
   if (!static_xinv(s, fx, S, dst, st)) {
     const long nrows = (long)s.ny * s.nz;

@@ -34,6 +34,9 @@ WorkspaceLayout workspace_layout(const ConvShape& s);
 // Validates the shape; throws Error on failure.
 void validate_shape(const ConvShape& s);
 
+// One-time kernel attribute set-up (safe to call repeatedly).
+void driver_init();
+
 // FFT description for length L on the current device (tables cached).
 FftDesc fft_desc(int L);
 
@@ -74,6 +77,13 @@ void enqueue_convolution(const ConvShape& s, float* d_out, const float* d_image,
 //   inverse_yx  S -> real volume (inverse y-FFT, x-C2R), unnormalised
 void forward_xy(const ConvShape& s, const float* src, float2* S, const float* kernel, float2* Kx,
                 float2* Kxy, hipStream_t stream);
+// the single passes forward_xy / inverse_yx are made of
+void pass_x_forward(const ConvShape& s, const float* src, float2* S, const float* kernel,
+                    float2* Kx, hipStream_t stream);
+void pass_y_forward(const ConvShape& s, float2* S, const float2* Kx, float2* Kxy,
+                    hipStream_t stream);
+void pass_y_inverse(const ConvShape& s, float2* S, hipStream_t stream);
+void pass_x_inverse(const ConvShape& s, const float2* S, float* dst, hipStream_t stream);
 void z_pass(const ConvShape& s, float2* S, const float2* operand, bool dense, bool conj,
             hipStream_t stream);
 void inverse_yx(const ConvShape& s, float2* S, float* dst, hipStream_t stream);

@@ -12,7 +12,8 @@
 //   fused z    "t<TX>-fft"       kernel column by the same FFT as the image
 //              "t<TX>-dft"       kernel spectrum by direct DFT over its kz
 //                                taps straight into the last-stage registers
-//              "t<TX>-dft-pipe"  the same, persistent and pipelined
+// and one batched form per pass (sb_*) whose workgroups take their
+// (argument record, tile) from a device task table.
 #pragma once
 
 #include "fft_static.hpp"
@@ -57,16 +58,9 @@ struct XRow {
 // circularly so tap x lands at (x - kx/2) mod nx.
 // ---------------------------------------------------------------------------
 template <int L, int PAIRS>
-__global__ void __launch_bounds__(PAIRS * threads_per_transform<L>())
-sk_xfwd(XPass2 a2) {
+__device__ __forceinline__ void xfwd_body(const XPassArgs& a, long blk, float2* lds) {
   using P = PlanOf<L, false>;
   constexpr int T = P::T, LP = XRow<L>::LP;
-  __shared__ float2 lds[PAIRS * LP];
-  // the kernel's rows take the first k_blocks workgroups so they run
-  // alongside the image rows instead of trailing them
-  const bool main = (long)blockIdx.x >= a2.k_blocks;
-  const XPassArgs& a = main ? a2.m : a2.k;
-  const long blk = main ? (long)blockIdx.x - a2.k_blocks : (long)blockIdx.x;
   const int t = threadIdx.x % T, c = threadIdx.x / T;
   const long ra = 2 * (blk * PAIRS + c), rb = ra + 1;
   const bool oka = ra < a.nrows, okb = rb < a.nrows;
@@ -99,17 +93,35 @@ sk_xfwd(XPass2 a2) {
   }
 }
 
+template <int L, int PAIRS>
+__global__ void __launch_bounds__(PAIRS * threads_per_transform<L>())
+sk_xfwd(XPass2 a2) {
+  __shared__ float2 lds[PAIRS * XRow<L>::LP];
+  // the kernel's rows take the first k_blocks workgroups so they run
+  // alongside the image rows instead of trailing them
+  const bool main = (long)blockIdx.x >= a2.k_blocks;
+  xfwd_body<L, PAIRS>(main ? a2.m : a2.k, main ? (long)blockIdx.x - a2.k_blocks : (long)blockIdx.x,
+                      lds);
+}
+
+// batched: workgroup i serves tasks[i] = {argument record, row-pair group}
+template <int L, int PAIRS>
+__global__ void __launch_bounds__(PAIRS * threads_per_transform<L>())
+sb_xfwd(const XPassArgs* __restrict__ args, const int2* __restrict__ tasks) {
+  __shared__ float2 lds[PAIRS * XRow<L>::LP];
+  const int2 tk = tasks[blockIdx.x];
+  xfwd_body<L, PAIRS>(args[tk.x], tk.y, lds);
+}
+
 // ---------------------------------------------------------------------------
 // x pass inverse (C2R), two rows per complex FFT
 // ---------------------------------------------------------------------------
 template <int L, int PAIRS>
-__global__ void __launch_bounds__(PAIRS * threads_per_transform<L>())
-sk_xinv(XPassArgs a) {
+__device__ __forceinline__ void xinv_body(const XPassArgs& a, long blk, float2* lds) {
   using P = PlanOf<L, false>;
   constexpr int T = P::T, LP = XRow<L>::LP;
-  __shared__ float2 lds[PAIRS * LP];
   const int t = threadIdx.x % T, c = threadIdx.x / T;
-  const long ra = 2 * ((long)blockIdx.x * PAIRS + c), rb = ra + 1;
+  const long ra = 2 * (blk * PAIRS + c), rb = ra + 1;
   const bool oka = ra < a.nrows, okb = rb < a.nrows;
   const float2* pa = a.csrc + ra * a.src_row;
   const float2* pb = a.csrc + rb * a.src_row;
@@ -142,9 +154,24 @@ sk_xinv(XPassArgs a) {
   });
 }
 
+template <int L, int PAIRS>
+__global__ void __launch_bounds__(PAIRS * threads_per_transform<L>())
+sk_xinv(XPassArgs a) {
+  __shared__ float2 lds[PAIRS * XRow<L>::LP];
+  xinv_body<L, PAIRS>(a, blockIdx.x, lds);
+}
+
+template <int L, int PAIRS>
+__global__ void __launch_bounds__(PAIRS * threads_per_transform<L>())
+sb_xinv(const XPassArgs* __restrict__ args, const int2* __restrict__ tasks) {
+  __shared__ float2 lds[PAIRS * XRow<L>::LP];
+  const int2 tk = tasks[blockIdx.x];
+  xinv_body<L, PAIRS>(args[tk.x], tk.y, lds);
+}
+
 // ---------------------------------------------------------------------------
-// column pass (y): tiles of TX bins x L positions over a 1D tile index
-// (image tiles first, then the kernel planes' tiles)
+// column pass (y): tiles of TX bins x L positions; tile index
+// o * ntx + xt with ntx = ceil(nbins / TX)
 // ---------------------------------------------------------------------------
 template <int L, int TX>
 struct ColTile {
@@ -154,17 +181,13 @@ struct ColTile {
   long base_src, base_dst;
   bool ok;
 
-  __device__ ColTile(const Col2& a2, long tile, int c) {
-    // kernel-plane tiles first, then the image's
-    const long ktiles = a2.ntiles - a2.m_tiles;
-    const bool main = tile >= ktiles;
-    a = main ? &a2.m : &a2.k;
-    const long tl = main ? tile - ktiles : tile;
-    const long o = tl / a2.ntx;
-    const int x = (int)(tl - o * a2.ntx) * TX + c;
-    ok = x < a->nbins;
-    base_src = o * a->src_outer + x;
-    base_dst = o * a->dst_outer + x;
+  __device__ ColTile(const ColArgs& args, long tile, int c) : a(&args) {
+    const int ntx = (args.nbins + TX - 1) / TX;
+    const long o = tile / ntx;
+    const int x = (int)(tile - o * ntx) * TX + c;
+    ok = x < args.nbins;
+    base_src = o * args.src_outer + x;
+    base_dst = o * args.dst_outer + x;
   }
   __device__ void load(int t, float2 (&v)[MAXE]) const {
     const float2* src = a->src + base_src;
@@ -187,18 +210,41 @@ struct ColTile {
   }
 };
 
+// kernel-plane tiles first, then the image's
+__device__ __forceinline__ const ColArgs& col2_args(const Col2& a2, long tile, long* local) {
+  const long ktiles = a2.ntiles - a2.m_tiles;
+  const bool main = tile >= ktiles;
+  *local = main ? tile - ktiles : tile;
+  return main ? a2.m : a2.k;
+}
+
 template <int L, int TX, int DIR>
-__global__ void __launch_bounds__(TX * threads_per_transform<L>())
-sk_col(Col2 a2) {
+__device__ __forceinline__ void col_body(const ColArgs& args, long local, float2* lds) {
   using CT = ColTile<L, TX>;
-  __shared__ float2 lds[L * TX];
   const int c = threadIdx.x % TX, t = threadIdx.x / TX;
-  const CT tile(a2, blockIdx.x, c);
+  const CT tile(args, local, c);
   float2 v[CT::MAXE];
   tile.load(t, v);
   auto addr = [&](int pos) { return pos * TX + c; };
-  sfft_run<L, false, DIR>(v, t, lds, addr, tile.a->fft.stw);
+  sfft_run<L, false, DIR>(v, t, lds, addr, args.fft.stw);
   tile.store(t, v);
+}
+
+template <int L, int TX, int DIR>
+__global__ void __launch_bounds__(TX * threads_per_transform<L>())
+sk_col(Col2 a2) {
+  __shared__ float2 lds[L * TX];
+  long local;
+  const ColArgs& a = col2_args(a2, blockIdx.x, &local);
+  col_body<L, TX, DIR>(a, local, lds);
+}
+
+template <int L, int TX, int DIR>
+__global__ void __launch_bounds__(TX * threads_per_transform<L>())
+sb_col(const ColArgs* __restrict__ args, const int2* __restrict__ tasks) {
+  __shared__ float2 lds[L * TX];
+  const int2 tk = tasks[blockIdx.x];
+  col_body<L, TX, DIR>(args[tk.x], tk.y, lds);
 }
 
 template <int L, int TX, int DIR>
@@ -210,12 +256,21 @@ sk_col_pipe(Col2 a2) {
   auto addr = [&](int pos) { return pos * TX + c; };
   long tile = blockIdx.x;
   float2 v[CT::MAXE], nv[CT::MAXE];
-  if (tile < a2.ntiles) CT(a2, tile, c).load(t, v);
+  long local;
+  if (tile < a2.ntiles) {
+    const ColArgs& a = col2_args(a2, tile, &local);
+    CT(a, local, c).load(t, v);
+  }
   for (; tile < a2.ntiles; tile += gridDim.x) {
     const long next = tile + gridDim.x;
-    if (next < a2.ntiles) CT(a2, next, c).load(t, nv);
-    const CT cur(a2, tile, c);
-    sfft_run<L, false, DIR>(v, t, lds, addr, cur.a->fft.stw);
+    if (next < a2.ntiles) {
+      long nl;
+      const ColArgs& na = col2_args(a2, next, &nl);
+      CT(na, nl, c).load(t, nv);
+    }
+    const ColArgs& a = col2_args(a2, tile, &local);
+    const CT cur(a, local, c);
+    sfft_run<L, false, DIR>(v, t, lds, addr, a.fft.stw);
     cur.store(t, v);
 #pragma unroll
     for (int i = 0; i < CT::MAXE; ++i) v[i] = nv[i];
@@ -282,7 +337,8 @@ struct ZTile {
   float2* col;
   const float2* kcol;
   bool ok;
-  __device__ ZTile(const ZArgs& a, long tile, int ntx, int c) {
+  __device__ ZTile(const ZArgs& a, long tile, int c) {
+    const int ntx = (a.nbins + TX - 1) / TX;
     const long y = tile / ntx;
     const int x = (int)(tile - y * ntx) * TX + c;
     ok = x < a.nbins;
@@ -317,13 +373,12 @@ __device__ __forceinline__ void zproduct(float2 (&v)[MAXE], const float2 (&w)[MA
   });
 }
 
+// kernel column by the same FFT as the image column
 template <int L, int TX>
-__global__ void __launch_bounds__(TX * threads_per_transform<L>())
-sk_zfused_fft(ZArgs a, int ntx) {
+__device__ __forceinline__ void zfused_fft_body(const ZArgs& a, long tile_index, float2* lds) {
   using ZT = ZTile<L, TX>;
-  __shared__ float2 lds[L * TX];
   const int c = threadIdx.x % TX, t = threadIdx.x / TX;
-  const ZT tile(a, blockIdx.x, ntx, c);
+  const ZT tile(a, tile_index, c);
   auto addr = [&](int pos) { return pos * TX + c; };
   float2 v[ZT::MAXE];
   tile.load(a, t, v);
@@ -352,11 +407,26 @@ sk_zfused_fft(ZArgs a, int ntx) {
 
 template <int L, int TX>
 __global__ void __launch_bounds__(TX * threads_per_transform<L>())
-sk_zfused_dft(ZArgs a, int ntx) {
+sk_zfused_fft(ZArgs a) {
+  __shared__ float2 lds[L * TX];
+  zfused_fft_body<L, TX>(a, blockIdx.x, lds);
+}
+
+template <int L, int TX>
+__global__ void __launch_bounds__(TX * threads_per_transform<L>())
+sb_zfused(const ZArgs* __restrict__ args, const int2* __restrict__ tasks) {
+  __shared__ float2 lds[L * TX];
+  const int2 tk = tasks[blockIdx.x];
+  zfused_fft_body<L, TX>(args[tk.x], tk.y, lds);
+}
+
+template <int L, int TX>
+__global__ void __launch_bounds__(TX * threads_per_transform<L>())
+sk_zfused_dft(ZArgs a) {
   using ZT = ZTile<L, TX>;
   __shared__ float2 lds[L * TX];
   const int c = threadIdx.x % TX, t = threadIdx.x / TX;
-  const ZT tile(a, blockIdx.x, ntx, c);
+  const ZT tile(a, blockIdx.x, c);
   auto addr = [&](int pos) { return pos * TX + c; };
   float2 v[ZT::MAXE];
   tile.load(a, t, v);
@@ -366,31 +436,6 @@ sk_zfused_dft(ZArgs a, int ntx) {
   zproduct<L>(v, w, t, a.conj != 0, a.scale);
   sfft_run<L, true, +1>(v, t, lds, addr, a.fft.stw);
   tile.store(a, t, v);
-}
-
-template <int L, int TX>
-__global__ void __launch_bounds__(TX * threads_per_transform<L>())
-sk_zfused_dft_pipe(ZArgs a, int ntx, long ntiles) {
-  using ZT = ZTile<L, TX>;
-  __shared__ float2 lds[L * TX];
-  const int c = threadIdx.x % TX, t = threadIdx.x / TX;
-  auto addr = [&](int pos) { return pos * TX + c; };
-  long tile = blockIdx.x;
-  float2 v[ZT::MAXE], nv[ZT::MAXE];
-  if (tile < ntiles) ZT(a, tile, ntx, c).load(a, t, v);
-  for (; tile < ntiles; tile += gridDim.x) {
-    const long next = tile + gridDim.x;
-    if (next < ntiles) ZT(a, next, ntx, c).load(a, t, nv);
-    const ZT cur(a, tile, ntx, c);
-    sfft_run<L, false, -1>(v, t, lds, addr, a.fft.stw);
-    float2 w[ZT::MAXE];
-    kernel_spectrum_direct<L>(w, t, cur.ok, cur.kcol, a.k_plane, a.nvalid, a.center, a.fft.tw);
-    zproduct<L>(v, w, t, a.conj != 0, a.scale);
-    sfft_run<L, true, +1>(v, t, lds, addr, a.fft.stw);
-    cur.store(a, t, v);
-#pragma unroll
-    for (int i = 0; i < ZT::MAXE; ++i) v[i] = nv[i];
-  }
 }
 
 // ---------------------------------------------------------------------------
@@ -416,13 +461,19 @@ bool sk_launch_xinv(const XPassArgs& a, hipStream_t st) {
   return true;
 }
 
-template <int L, int WANT>
-bool sk_launch_col(const Col2& a, int dir, hipStream_t st) {
-  constexpr int TX = fit_tile<L, WANT>();
+template <int L, int TX>
+Col2 col2_tiles(const Col2& a) {
   Col2 b = a;
   b.ntx = (a.m.nbins + TX - 1) / TX;
   b.m_tiles = (long)b.ntx * a.m_outer;
   b.ntiles = b.m_tiles + (long)b.ntx * a.k_outer;
+  return b;
+}
+
+template <int L, int WANT>
+bool sk_launch_col(const Col2& a, int dir, hipStream_t st) {
+  constexpr int TX = fit_tile<L, WANT>();
+  const Col2 b = col2_tiles<L, TX>(a);
   const dim3 block(TX * threads_per_transform<L>());
   if (dir < 0)
     hipLaunchKernelGGL((sk_col<L, TX, -1>), dim3((unsigned)b.ntiles), block, 0, st, b);
@@ -435,10 +486,7 @@ template <int L, int WANT>
 bool sk_launch_col_pipe(const Col2& a, int dir, hipStream_t st) {
   constexpr int TX = fit_tile<L, WANT>();
   constexpr int threads = TX * threads_per_transform<L>();
-  Col2 b = a;
-  b.ntx = (a.m.nbins + TX - 1) / TX;
-  b.m_tiles = (long)b.ntx * a.m_outer;
-  b.ntiles = b.m_tiles + (long)b.ntx * a.k_outer;
+  const Col2 b = col2_tiles<L, TX>(a);
   const void* fn = dir < 0 ? (const void*)sk_col_pipe<L, TX, -1> : (const void*)sk_col_pipe<L, TX, +1>;
   long grid = persistent_blocks(fn, threads);
   if (grid > b.ntiles) grid = b.ntiles;
@@ -455,7 +503,7 @@ bool sk_launch_zfused_fft(const ZArgs& a, long ny, hipStream_t st) {
   constexpr int TX = fit_tile<L, WANT>();
   const int ntx = (a.nbins + TX - 1) / TX;
   hipLaunchKernelGGL((sk_zfused_fft<L, TX>), dim3((unsigned)(ntx * ny)),
-                     dim3(TX * threads_per_transform<L>()), 0, st, a, ntx);
+                     dim3(TX * threads_per_transform<L>()), 0, st, a);
   return true;
 }
 
@@ -472,23 +520,37 @@ bool sk_launch_zfused_dft(const ZArgs& a, long ny, hipStream_t st) {
   constexpr int TX = fit_tile<L, WANT>();
   const int ntx = (a.nbins + TX - 1) / TX;
   hipLaunchKernelGGL((sk_zfused_dft<L, TX>), dim3((unsigned)(ntx * ny)),
-                     dim3(TX * threads_per_transform<L>()), 0, st, a, ntx);
+                     dim3(TX * threads_per_transform<L>()), 0, st, a);
   return true;
 }
 
-template <int L, int WANT>
-bool sk_launch_zfused_dft_pipe(const ZArgs& a, long ny, hipStream_t st) {
-  if (a.nvalid > dft_kz_limit<L>()) return false;
-  constexpr int TX = fit_tile<L, WANT>();
-  constexpr int threads = TX * threads_per_transform<L>();
-  const int ntx = (a.nbins + TX - 1) / TX;
-  const long ntiles = (long)ntx * ny;
-  long grid = persistent_blocks((const void*)sk_zfused_dft_pipe<L, TX>, threads);
-  if (grid > ntiles) grid = ntiles;
-  if (grid < 1) grid = 1;
-  hipLaunchKernelGGL((sk_zfused_dft_pipe<L, TX>), dim3((unsigned)grid), dim3(threads), 0, st, a,
-                     ntx, ntiles);
-  return true;
+// batched launches over device task tables (default variants)
+template <int L>
+void sb_launch_xfwd(const XPassArgs* args, const int2* tasks, int ntasks, hipStream_t st) {
+  constexpr int P = fit_pairs<L, 4>();
+  hipLaunchKernelGGL((sb_xfwd<L, P>), dim3((unsigned)ntasks), dim3(P * threads_per_transform<L>()),
+                     0, st, args, tasks);
+}
+template <int L>
+void sb_launch_xinv(const XPassArgs* args, const int2* tasks, int ntasks, hipStream_t st) {
+  constexpr int P = fit_pairs<L, 4>();
+  hipLaunchKernelGGL((sb_xinv<L, P>), dim3((unsigned)ntasks), dim3(P * threads_per_transform<L>()),
+                     0, st, args, tasks);
+}
+template <int L>
+void sb_launch_col(const ColArgs* args, const int2* tasks, int ntasks, int dir, hipStream_t st) {
+  constexpr int TX = fit_tile<L, 16>();
+  const dim3 block(TX * threads_per_transform<L>());
+  if (dir < 0)
+    hipLaunchKernelGGL((sb_col<L, TX, -1>), dim3((unsigned)ntasks), block, 0, st, args, tasks);
+  else
+    hipLaunchKernelGGL((sb_col<L, TX, +1>), dim3((unsigned)ntasks), block, 0, st, args, tasks);
+}
+template <int L>
+void sb_launch_zfused(const ZArgs* args, const int2* tasks, int ntasks, hipStream_t st) {
+  constexpr int TX = fit_tile<L, 16>();
+  hipLaunchKernelGGL((sb_zfused<L, TX>), dim3((unsigned)ntasks), dim3(TX * threads_per_transform<L>()),
+                     0, st, args, tasks);
 }
 
 template <int L>
@@ -499,10 +561,13 @@ StaticEntry make_static_entry() {
   e.col = {{"t16", &sk_launch_col<L, 16>},
            {"t16-pipe", &sk_launch_col_pipe<L, 16>},
            {"t8-pipe", &sk_launch_col_pipe<L, 8>}};
-  e.zfused = {{"t16-fft", &sk_launch_zfused_fft<L, 16>},
-              {"t16-dft", &sk_launch_zfused_dft<L, 16>},
-              {"t16-dft-pipe", &sk_launch_zfused_dft_pipe<L, 16>},
-              {"t8-dft-pipe", &sk_launch_zfused_dft_pipe<L, 8>}};
+  e.zfused = {{"t16-fft", &sk_launch_zfused_fft<L, 16>}, {"t16-dft", &sk_launch_zfused_dft<L, 16>}};
+  e.batch.pairs = fit_pairs<L, 4>();
+  e.batch.tx = fit_tile<L, 16>();
+  e.batch.xfwd = &sb_launch_xfwd<L>;
+  e.batch.xinv = &sb_launch_xinv<L>;
+  e.batch.col = &sb_launch_col<L>;
+  e.batch.zfused = &sb_launch_zfused<L>;
   return e;
 }
 

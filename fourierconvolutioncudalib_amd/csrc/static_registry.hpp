@@ -25,12 +25,23 @@ struct Variant {
   F fn;
 };
 
+// batched forms: one launch over a task table {argument record, tile}
+struct StaticBatch {
+  int pairs = 1;  // row pairs per workgroup of the x passes
+  int tx = 1;     // bins per tile of the column / z passes
+  void (*xfwd)(const XPassArgs*, const int2*, int, hipStream_t) = nullptr;
+  void (*xinv)(const XPassArgs*, const int2*, int, hipStream_t) = nullptr;
+  void (*col)(const ColArgs*, const int2*, int, int dir, hipStream_t) = nullptr;
+  void (*zfused)(const ZArgs*, const int2*, int, hipStream_t) = nullptr;
+};
+
 struct StaticEntry {
   std::vector<Variant<XFwdFn>> xfwd;
   std::vector<Variant<XInvFn>> xinv;
   std::vector<Variant<ColFn>> col;
   std::vector<Variant<ZFn>> zfused;
   int selected[kSNumPasses] = {0, 0, 0, 0};
+  StaticBatch batch;
   int count(int pass) const {
     switch (pass) {
       case kSxFwd: return (int)xfwd.size();

@@ -102,6 +102,26 @@ CFG2 = dict(name="lightsheet_600x500x240_k31x31x61", image=(240, 500, 600), kern
             sigma=(2.0, 2.0, 8.0), seed=1002)
 
 
+MENU = (64, 80, 96, 105, 112, 120, 128, 144, 160, 192, 210, 224, 240, 256)
+
+
+def uneven_block_shapes(seed: int, count: int = 64, menu=MENU, kmin=9, kmax=41):
+    """(image C-order shape, kernel C-order shape, psf seed) per block: every
+    image extent drawn from `menu`, every kernel extent odd in [kmin, kmax]
+    and no larger than the image extent."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for b in range(count):
+        shape = tuple(int(rng.choice(menu)) for _ in range(3))
+        kshape = []
+        for n in shape:
+            hi = min(kmax, n if n % 2 else n - 1)
+            k = int(rng.integers(kmin // 2, hi // 2 + 1)) * 2 + 1
+            kshape.append(min(k, hi))
+        out.append((shape, tuple(kshape), seed * 1000 + b))
+    return out
+
+
 def workload(cfg):
     """(image, psf) float32 arrays for a named uniform or blob workload."""
     if cfg is CFG2:

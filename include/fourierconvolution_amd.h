@@ -57,6 +57,26 @@ FCAMD_API int fcamd_conv_grad_kernel_device(float* d_dkernel, const float* d_gra
                                             const int* kernelDim, void* d_workspace,
                                             long long workspace_bytes, void* stream);
 
+/* Batched uneven blocks (no counterpart in the reference): `nblocks`
+ * convolutions, block b with dims imDims[3b..3b+2] ({x,y,z}), kernel dims
+ * kernelDims[3b..3b+2], device buffers d_images[b], d_kernels[b] and
+ * d_outs[b] (may alias the image).  The plan groups the blocks of every pass
+ * by transform length (one launch per length, length groups on forked
+ * streams); fcamd_batch_execute enqueues the sequence on `stream`,
+ * fcamd_batch_graph_launch captures it into a hipGraph on the first call and
+ * replays that graph afterwards.  The buffers are bound at creation.
+ * d_workspace may be NULL (the plan allocates it) or hold
+ * fcamd_batch_workspace_bytes() bytes.  create returns NULL on error. */
+FCAMD_API long long fcamd_batch_workspace_bytes(int nblocks, const int* imDims,
+                                                const int* kernelDims);
+FCAMD_API void* fcamd_batch_create(int nblocks, const int* imDims, const int* kernelDims,
+                                   float* const* d_outs, const float* const* d_images,
+                                   const float* const* d_kernels, int op, void* d_workspace,
+                                   long long workspace_bytes);
+FCAMD_API int fcamd_batch_execute(void* plan, void* stream);
+FCAMD_API int fcamd_batch_graph_launch(void* plan, void* stream);
+FCAMD_API void fcamd_batch_destroy(void* plan);
+
 /* Per-pass HIP-event timing of the convolution passes (diagnostics and the
  * benchmark's roofline figures).  While enabled every pass launch is
  * bracketed by two events on its stream.  fcamd_pass_times waits for them,
