@@ -140,6 +140,34 @@ def batched_comparison(fc, torch, reps=5):
     }
 
 
+def rl_comparison(fc, torch):
+    """Multiview Richardson-Lucy, 4 views of 384x384x192 with 25x25x51 PSFs,
+    10 iterations: fused pass sequence vs per-view convolutions."""
+    from fourierconvolutioncudalib_amd import synth
+    from fourierconvolutioncudalib_amd.rl import richardson_lucy
+    cfg = synth.RL_CFG
+    _, views, psfs = synth.multiview(cfg["image"], cfg["kernel"], cfg["views"], cfg["seed"])
+    tv = [torch.from_numpy(v).cuda() for v in views]
+    tk = [torch.from_numpy(k).cuda() for k in psfs]
+    res = {"workload": cfg["name"], "iterations": cfg["iterations"]}
+    for fused in (True, False):
+        richardson_lucy(tv, tk, iterations=1, fused=fused)
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        richardson_lucy(tv, tk, iterations=cfg["iterations"], fused=fused)
+        e1.record()
+        torch.cuda.synchronize()
+        res["fused_ms" if fused else "per_view_ms"] = e0.elapsed_time(e1)
+    res["speedup_fused_vs_per_view"] = res["per_view_ms"] / res["fused_ms"]
+    n = 1
+    for d in cfg["image"]:
+        n *= d
+    res["view_voxel_updates_per_s"] = cfg["views"] * n * cfg["iterations"] / (res["fused_ms"] * 1e-3)
+    return res
+
+
 def load_traffic(workload):
     path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
     if not os.path.exists(path):
@@ -287,6 +315,7 @@ def main():
     }
     if not args.no_batched:
         result["batched"] = batched_comparison(fc, torch)
+        result["richardson_lucy"] = rl_comparison(fc, torch)
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(img_np, psf_np)
     else:

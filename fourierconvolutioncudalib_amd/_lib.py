@@ -57,6 +57,10 @@ SIGNATURES = {
     "fcamd_batch_execute": (_i, [_vp, _vp]),
     "fcamd_batch_graph_launch": (_i, [_vp, _vp]),
     "fcamd_batch_destroy": (None, [_vp]),
+    "fcamd_rl_workspace_bytes": (_ll, [_i, _ip, _ip, _i]),
+    "fcamd_rl_fused_available": (_i, [_i, _ip, _ip]),
+    "fcamd_rl_multiview_device": (_i, [_vp, _vp, _i, _ip, _vp, _ip, _i, ctypes.c_float, _i, _i,
+                                       _vp, _ll, _vp]),
     "fcamd_pass_timing": (None, [_i]),
     "fcamd_pass_times": (_i, [ctypes.POINTER(ctypes.c_double), _ip, _i]),
     "fcamd_pass_name": (ctypes.c_char_p, [_i]),

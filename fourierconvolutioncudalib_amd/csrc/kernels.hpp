@@ -71,6 +71,36 @@ struct Col2 {
   long ntiles;      // total tiles
 };
 
+// fused multiview Richardson-Lucy passes (rl_kernels.hpp)
+constexpr int kMaxViews = 8;
+
+struct RLZArgs {
+  FftDesc fft;  // length nz
+  int nbins;
+  long s_row, s_plane;  // strides of y and z (all spectra share the layout)
+  int nviews;
+  float scale;                       // 1 / N
+  const float2* src;                 // fan-out input (FFTxy of psi)
+  float2* dst;                       // sum output
+  float2* view[kMaxViews];           // per-view spectra (fan-out out / sum in)
+  const float2* kplanes[kMaxViews];  // per-view kernel planes Kxy
+  int kz[kMaxViews];
+  int cz[kMaxViews];
+};
+
+struct RLXArgs {
+  FftDesc fft;  // length nx
+  long nrows;
+  int nbins;
+  long srow;  // spectrum row stride (nxp)
+  long rrow;  // real row stride (nx)
+  float eps;
+  float inv_views;
+  float2* spec[kMaxViews];       // per-view spectra (xratio) / summed spectrum (xupdate: [0])
+  const float* phi[kMaxViews];   // per-view data (xratio)
+  float* psi;                    // estimate (xupdate, read and written)
+};
+
 void launch_gk_xfwd(const XPassArgs& a, long npairs, hipStream_t st);
 void launch_gk_xinv(const XPassArgs& a, long npairs, hipStream_t st);
 void launch_gk_col(const ColArgs& a, int dir, long nouter, hipStream_t st);

@@ -43,6 +43,57 @@ gather_kernel_support(const float* __restrict__ u, float* __restrict__ dk, int n
   }
 }
 
+// Richardson-Lucy elementwise steps (unfused iteration and initialisation)
+__global__ void __launch_bounds__(256)
+rl_mean_views(float* __restrict__ psi, const float* const* __restrict__ views, int nviews, long n) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    float s = 0.f;
+    for (int v = 0; v < nviews; ++v) s += views[v][i];
+    psi[i] = s / (float)nviews;
+  }
+}
+
+// ratio = phi / max(blur, eps)
+__global__ void __launch_bounds__(256)
+rl_ratio(float* __restrict__ ratio, const float* __restrict__ phi, const float* __restrict__ blur,
+         float eps, long n) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    ratio[i] = phi[i] / fmaxf(blur[i], eps);
+}
+
+// acc = (first ? 0 : acc) + back
+__global__ void __launch_bounds__(256)
+rl_accumulate(float* __restrict__ acc, const float* __restrict__ back, int first, long n) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    acc[i] = first ? back[i] : acc[i] + back[i];
+}
+
+// psi = psi * acc * inv_views
+__global__ void __launch_bounds__(256)
+rl_update(float* __restrict__ psi, const float* __restrict__ acc, float inv_views, long n) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    psi[i] = psi[i] * acc[i] * inv_views;
+}
+
+static unsigned ew_blocks(long n) {
+  long b = (n + 255) / 256;
+  return (unsigned)(b > 8192 ? 8192 : (b < 1 ? 1 : b));
+}
+
+void launch_rl_mean(float* psi, const float* const* d_views, int nviews, long n, hipStream_t st) {
+  hipLaunchKernelGGL(rl_mean_views, dim3(ew_blocks(n)), dim3(256), 0, st, psi, d_views, nviews, n);
+}
+void launch_rl_ratio(float* ratio, const float* phi, const float* blur, float eps, long n,
+                     hipStream_t st) {
+  hipLaunchKernelGGL(rl_ratio, dim3(ew_blocks(n)), dim3(256), 0, st, ratio, phi, blur, eps, n);
+}
+void launch_rl_accumulate(float* acc, const float* back, int first, long n, hipStream_t st) {
+  hipLaunchKernelGGL(rl_accumulate, dim3(ew_blocks(n)), dim3(256), 0, st, acc, back, first, n);
+}
+void launch_rl_update(float* psi, const float* acc, float inv_views, long n, hipStream_t st) {
+  hipLaunchKernelGGL(rl_update, dim3(ew_blocks(n)), dim3(256), 0, st, psi, acc, inv_views, n);
+}
+
 }  // namespace fcamd
 
 #include "conv_driver.hpp"

@@ -1,0 +1,229 @@
+// rl_kernels.hpp -- compile-time-length kernels of the fused multiview
+// Richardson-Lucy iteration
+//   psi <- psi * (1/V) sum_v corr(phi_v / max(conv(psi, P_v), eps), P_v)
+// built on fft_static.hpp and the pass bodies of static_kernels_impl.hpp.
+//
+//   sk_zfan      one z tile of FFTxy(psi): FFTz once, then per view the
+//                product with FFTz(P_v) and IFFTz into that view's spectrum
+//   sk_xratio    per view, rows of the blurred estimate: x-C2R, ratio
+//                phi / max(blur, eps) in registers, x-R2C back in place
+//                (the ratio volume never reaches HBM)
+//   sk_zsum      one z tile: sum over views of FFTz(ratio_v) * conj(FFTz(P_v)),
+//                IFFTz (correlations add in the spectral domain)
+//   sk_xupdate   x-C2R of the summed correlation with the epilogue
+//                psi <- psi * acc / V
+#pragma once
+
+#include "static_kernels_impl.hpp"
+
+namespace fcamd {
+
+// loads the compact kernel column of view v (planes placed at (z - cz) mod L)
+template <int L, int MAXE>
+__device__ __forceinline__ void load_kernel_column(float2 (&w)[MAXE], int t, bool ok,
+                                                   const float2* kcol, long kplane, int kz, int cz) {
+  sfft_for_inputs<L, false>(t, [&](int slot, int pos) {
+    int j = pos + cz;
+    if (j >= L) j -= L;
+    float2 val = make_float2(0.f, 0.f);
+    if (ok && j < kz) val = kcol[(long)j * kplane];
+    w[slot] = val;
+  });
+}
+
+template <int L, int TX>
+__global__ void __launch_bounds__(TX * threads_per_transform<L>())
+sk_zfan(RLZArgs a) {
+  using P = PlanOf<L, false>;
+  __shared__ float2 lds[L * TX];
+  const int c = threadIdx.x % TX, t = threadIdx.x / TX;
+  const int ntx = (a.nbins + TX - 1) / TX;
+  const long y = blockIdx.x / ntx;
+  const int x = (int)(blockIdx.x - y * ntx) * TX + c;
+  const bool ok = x < a.nbins;
+  const long base = y * a.s_row + x;
+  auto addr = [&](int pos) { return pos * TX + c; };
+  float2 X[P::MAXE()];
+  sfft_for_inputs<L, false>(t, [&](int slot, int pos) {
+    float2 val = make_float2(0.f, 0.f);
+    if (ok) val = a.src[base + (long)pos * a.s_plane];
+    X[slot] = val;
+  });
+  sfft_run<L, false, -1>(X, t, lds, addr, a.fft.stw);
+  for (int v = 0; v < a.nviews; ++v) {
+    float2 W[P::MAXE()];
+    load_kernel_column<L>(W, t, ok, a.kplanes[v] + base, a.s_plane, a.kz[v], a.cz[v]);
+    sfft_run<L, false, -1>(W, t, lds, addr, a.fft.stw);
+    sfft_for_outputs<L, false>(t, [&](int slot, int) { W[slot] = cscale(cmul(X[slot], W[slot]), a.scale); });
+    sfft_run<L, true, +1>(W, t, lds, addr, a.fft.stw);
+    float2* out = a.view[v] + base;
+    if (ok) sfft_for_outputs<L, true>(t, [&](int slot, int pos) { out[(long)pos * a.s_plane] = W[slot]; });
+  }
+}
+
+template <int L, int TX>
+__global__ void __launch_bounds__(TX * threads_per_transform<L>())
+sk_zsum(RLZArgs a) {
+  using P = PlanOf<L, false>;
+  __shared__ float2 lds[L * TX];
+  const int c = threadIdx.x % TX, t = threadIdx.x / TX;
+  const int ntx = (a.nbins + TX - 1) / TX;
+  const long y = blockIdx.x / ntx;
+  const int x = (int)(blockIdx.x - y * ntx) * TX + c;
+  const bool ok = x < a.nbins;
+  const long base = y * a.s_row + x;
+  auto addr = [&](int pos) { return pos * TX + c; };
+  float2 acc[P::MAXE()];
+#pragma unroll
+  for (int i = 0; i < P::MAXE(); ++i) acc[i] = make_float2(0.f, 0.f);
+  for (int v = 0; v < a.nviews; ++v) {
+    float2 X[P::MAXE()], W[P::MAXE()];
+    const float2* in = a.view[v] + base;
+    sfft_for_inputs<L, false>(t, [&](int slot, int pos) {
+      float2 val = make_float2(0.f, 0.f);
+      if (ok) val = in[(long)pos * a.s_plane];
+      X[slot] = val;
+    });
+    load_kernel_column<L>(W, t, ok, a.kplanes[v] + base, a.s_plane, a.kz[v], a.cz[v]);
+    sfft_run<L, false, -1>(X, t, lds, addr, a.fft.stw);
+    sfft_run<L, false, -1>(W, t, lds, addr, a.fft.stw);
+    sfft_for_outputs<L, false>(t, [&](int slot, int) { acc[slot] = cadd(acc[slot], cmulc(X[slot], W[slot])); });
+  }
+  sfft_for_outputs<L, false>(t, [&](int slot, int) { acc[slot] = cscale(acc[slot], a.scale); });
+  sfft_run<L, true, +1>(acc, t, lds, addr, a.fft.stw);
+  float2* out = a.dst + base;
+  if (ok) sfft_for_outputs<L, true>(t, [&](int slot, int pos) { out[(long)pos * a.s_plane] = acc[slot]; });
+}
+
+// x-C2R (reversed plan: its outputs sit exactly at the forward plan's
+// stage-0 input positions), ratio, x-R2C (forward plan), separation.
+template <int L, int PAIRS>
+__global__ void __launch_bounds__(PAIRS * threads_per_transform<L>())
+sk_xratio(RLXArgs a) {
+  using P = PlanOf<L, false>;
+  constexpr int T = P::T, LP = XRow<L>::LP;
+  __shared__ float2 lds[PAIRS * LP];
+  const long groups = ((a.nrows + 1) / 2 + PAIRS - 1) / PAIRS;
+  const int v = (int)(blockIdx.x / groups);
+  const long blk = blockIdx.x - (long)v * groups;
+  const int t = threadIdx.x % T, c = threadIdx.x / T;
+  const long ra = 2 * (blk * PAIRS + c), rb = ra + 1;
+  const bool oka = ra < a.nrows, okb = rb < a.nrows;
+  float2* pa = a.spec[v] + ra * a.srow;
+  float2* pb = a.spec[v] + rb * a.srow;
+  const int nb = a.nbins;
+  float2 z[P::MAXE()];
+  sfft_for_inputs<L, true>(t, [&](int slot, int pos) {
+    const bool mirror = pos >= nb;
+    const int kb = mirror ? L - pos : pos;
+    float2 A = make_float2(0.f, 0.f), B = make_float2(0.f, 0.f);
+    if (oka) A = pa[kb];
+    if (okb) B = pb[kb];
+    if (kb == 0 || 2 * kb == L) {
+      A.y = 0.f;
+      B.y = 0.f;
+    }
+    if (mirror) {
+      A.y = -A.y;
+      B.y = -B.y;
+    }
+    z[slot] = make_float2(A.x - B.y, A.y + B.x);
+  });
+  float2* row = lds + c * LP;
+  auto addr = [&](int pos) { return XRow<L>::at(pos); };
+  sfft_run<L, true, +1>(z, t, row, addr, a.fft.stw);
+  // real rows of the blurred estimate -> ratio rows (same positions)
+  const float* fa = a.phi[v] + ra * a.rrow;
+  const float* fb = a.phi[v] + rb * a.rrow;
+  const float eps = a.eps;
+  sfft_for_outputs<L, true>(t, [&](int slot, int pos) {
+    float ya = 0.f, yb = 0.f;
+    if (oka) ya = fa[pos] / fmaxf(z[slot].x, eps);
+    if (okb) yb = fb[pos] / fmaxf(z[slot].y, eps);
+    z[slot] = make_float2(ya, yb);
+  });
+  sfft_run<L, false, -1>(z, t, row, addr, a.fft.stw);
+  sfft_for_outputs<L, false>(t, [&](int slot, int pos) { row[addr(pos)] = z[slot]; });
+  __syncthreads();
+  for (int kb = t; kb < nb; kb += T) {
+    const float2 Z = row[addr(kb)];
+    const float2 Zm = row[addr(kb == 0 ? 0 : L - kb)];
+    if (oka) pa[kb] = make_float2(0.5f * (Z.x + Zm.x), 0.5f * (Z.y - Zm.y));
+    if (okb) pb[kb] = make_float2(0.5f * (Z.y + Zm.y), 0.5f * (Zm.x - Z.x));
+  }
+}
+
+template <int L, int PAIRS>
+__global__ void __launch_bounds__(PAIRS * threads_per_transform<L>())
+sk_xupdate(RLXArgs a) {
+  using P = PlanOf<L, false>;
+  constexpr int T = P::T, LP = XRow<L>::LP;
+  __shared__ float2 lds[PAIRS * LP];
+  const int t = threadIdx.x % T, c = threadIdx.x / T;
+  const long ra = 2 * ((long)blockIdx.x * PAIRS + c), rb = ra + 1;
+  const bool oka = ra < a.nrows, okb = rb < a.nrows;
+  const float2* pa = a.spec[0] + ra * a.srow;
+  const float2* pb = a.spec[0] + rb * a.srow;
+  const int nb = a.nbins;
+  float2 z[P::MAXE()];
+  sfft_for_inputs<L, false>(t, [&](int slot, int pos) {
+    const bool mirror = pos >= nb;
+    const int kb = mirror ? L - pos : pos;
+    float2 A = make_float2(0.f, 0.f), B = make_float2(0.f, 0.f);
+    if (oka) A = pa[kb];
+    if (okb) B = pb[kb];
+    if (kb == 0 || 2 * kb == L) {
+      A.y = 0.f;
+      B.y = 0.f;
+    }
+    if (mirror) {
+      A.y = -A.y;
+      B.y = -B.y;
+    }
+    z[slot] = make_float2(A.x - B.y, A.y + B.x);
+  });
+  float2* row = lds + c * LP;
+  auto addr = [&](int pos) { return XRow<L>::at(pos); };
+  sfft_run<L, false, +1>(z, t, row, addr, a.fft.stw);
+  float* qa = a.psi + ra * a.rrow;
+  float* qb = a.psi + rb * a.rrow;
+  const float iv = a.inv_views;
+  sfft_for_outputs<L, false>(t, [&](int slot, int pos) {
+    if (oka) qa[pos] = qa[pos] * z[slot].x * iv;
+    if (okb) qb[pos] = qb[pos] * z[slot].y * iv;
+  });
+}
+
+template <int L>
+void rl_launch_zfan(const RLZArgs& a, long ny, hipStream_t st) {
+  constexpr int TX = fit_tile<L, 16>();
+  const long ntx = (a.nbins + TX - 1) / TX;
+  hipLaunchKernelGGL((sk_zfan<L, TX>), dim3((unsigned)(ntx * ny)), dim3(TX * threads_per_transform<L>()),
+                     0, st, a);
+}
+
+template <int L>
+void rl_launch_zsum(const RLZArgs& a, long ny, hipStream_t st) {
+  constexpr int TX = fit_tile<L, 16>();
+  const long ntx = (a.nbins + TX - 1) / TX;
+  hipLaunchKernelGGL((sk_zsum<L, TX>), dim3((unsigned)(ntx * ny)), dim3(TX * threads_per_transform<L>()),
+                     0, st, a);
+}
+
+template <int L>
+void rl_launch_xratio(const RLXArgs& a, int nviews, hipStream_t st) {
+  constexpr int P = fit_pairs<L, 4>();
+  const long groups = ((a.nrows + 1) / 2 + P - 1) / P;
+  hipLaunchKernelGGL((sk_xratio<L, P>), dim3((unsigned)(groups * nviews)),
+                     dim3(P * threads_per_transform<L>()), 0, st, a);
+}
+
+template <int L>
+void rl_launch_xupdate(const RLXArgs& a, hipStream_t st) {
+  constexpr int P = fit_pairs<L, 4>();
+  const long groups = ((a.nrows + 1) / 2 + P - 1) / P;
+  hipLaunchKernelGGL((sk_xupdate<L, P>), dim3((unsigned)groups), dim3(P * threads_per_transform<L>()),
+                     0, st, a);
+}
+
+}  // namespace fcamd

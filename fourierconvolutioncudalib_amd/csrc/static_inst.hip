@@ -2,6 +2,7 @@
 // lengths (FC_GROUP, set by the Makefile) and registers them at load time.
 // The groups split the instantiations over several objects so they compile
 // in parallel.
+#include "rl_kernels.hpp"
 #include "static_kernels_impl.hpp"
 
 #ifndef FC_GROUP
@@ -31,7 +32,15 @@ namespace {
 
 struct Registrar {
   Registrar() {
-#define FC_REG(L) static_register(L, make_static_entry<L>());
+#define FC_REG(L)                                \
+  {                                              \
+    StaticEntry e = make_static_entry<L>();      \
+    e.rl.zfan = &rl_launch_zfan<L>;              \
+    e.rl.zsum = &rl_launch_zsum<L>;              \
+    e.rl.xratio = &rl_launch_xratio<L>;          \
+    e.rl.xupdate = &rl_launch_xupdate<L>;        \
+    static_register(L, e);                       \
+  }
     FC_LENGTHS(FC_REG)
 #undef FC_REG
   }

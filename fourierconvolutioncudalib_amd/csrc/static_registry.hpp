@@ -35,6 +35,14 @@ struct StaticBatch {
   void (*zfused)(const ZArgs*, const int2*, int, hipStream_t) = nullptr;
 };
 
+// fused Richardson-Lucy kernels (rl_kernels.hpp)
+struct StaticRL {
+  void (*zfan)(const RLZArgs&, long ny, hipStream_t) = nullptr;
+  void (*zsum)(const RLZArgs&, long ny, hipStream_t) = nullptr;
+  void (*xratio)(const RLXArgs&, int nviews, hipStream_t) = nullptr;
+  void (*xupdate)(const RLXArgs&, hipStream_t) = nullptr;
+};
+
 struct StaticEntry {
   std::vector<Variant<XFwdFn>> xfwd;
   std::vector<Variant<XInvFn>> xinv;
@@ -42,6 +50,7 @@ struct StaticEntry {
   std::vector<Variant<ZFn>> zfused;
   int selected[kSNumPasses] = {0, 0, 0, 0};
   StaticBatch batch;
+  StaticRL rl;
   int count(int pass) const {
     switch (pass) {
       case kSxFwd: return (int)xfwd.size();

@@ -122,6 +122,41 @@ def uneven_block_shapes(seed: int, count: int = 64, menu=MENU, kmin=9, kmax=41):
     return out
 
 
+def rotation_y(angle_deg: float) -> np.ndarray:
+    """Rotation of the PSF axes about the y axis (x, y, z order)."""
+    a = np.deg2rad(angle_deg)
+    c, s = np.cos(a), np.sin(a)
+    return np.array([[c, 0.0, s], [0.0, 1.0, 0.0], [-s, 0.0, c]])
+
+
+def multiview(shape_zyx, kshape_zyx, nviews: int, seed: int, sigma_xyz=(2.0, 2.0, 6.0),
+              blobs_count=600, noise=True):
+    """(phantom, views, psfs): a blob phantom seen through `nviews` PSFs whose
+    long (z) axis is rotated by 180/nviews degrees per view, each view the
+    circular convolution of the phantom with its PSF plus Poisson noise."""
+    phantom = blobs(shape_zyx, seed, count=blobs_count, background=10.0, poisson=False)
+    psfs = [gaussian_psf(kshape_zyx, sigma_xyz, rotation_y(180.0 * v / nviews))
+            for v in range(nviews)]
+    fp = np.fft.rfftn(phantom.astype(np.float64))
+    rng = np.random.default_rng(seed + 1)
+    views = []
+    for k in psfs:
+        pk = np.zeros(shape_zyx)
+        kz, ky, kx = k.shape
+        zi = (np.arange(kz) - kz // 2) % shape_zyx[0]
+        yi = (np.arange(ky) - ky // 2) % shape_zyx[1]
+        xi = (np.arange(kx) - kx // 2) % shape_zyx[2]
+        pk[np.ix_(zi, yi, xi)] = k
+        blur = np.fft.irfftn(fp * np.fft.rfftn(pk), s=shape_zyx, axes=(0, 1, 2))
+        blur = np.maximum(blur, 0.0)
+        views.append((rng.poisson(blur) if noise else blur).astype(np.float32))
+    return phantom, views, psfs
+
+
+RL_CFG = dict(name="rl_4x384x384x192_k25x25x51", image=(192, 384, 384), kernel=(51, 25, 25),
+              views=4, iterations=10, seed=1005)
+
+
 def workload(cfg):
     """(image, psf) float32 arrays for a named uniform or blob workload."""
     if cfg is CFG2:

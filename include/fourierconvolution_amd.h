@@ -77,6 +77,25 @@ FCAMD_API int fcamd_batch_execute(void* plan, void* stream);
 FCAMD_API int fcamd_batch_graph_launch(void* plan, void* stream);
 FCAMD_API void fcamd_batch_destroy(void* plan);
 
+/* Multiview Richardson-Lucy deconvolution (no counterpart in the
+ * reference; the iterative caller of the hot path):
+ *   psi <- psi * (1/V) sum_v corr(phi_v / max(conv(psi, P_v), eps), P_v)
+ * with conv / corr the convolution semantics of convolution3DfftCUDAInPlace
+ * and its adjoint.  V = nviews (1..8) views d_views[v] share the image dims
+ * imDim ({x,y,z}); view v's PSF d_kernels[v] has dims kernelDims[3v..3v+2].
+ * init_psi = 1 starts from the mean of the views, else from d_psi.  fused = 1
+ * uses the fused pass sequence when every axis length has compile-time
+ * kernels (fcamd_rl_fused_available), else the per-view form.  All pointers
+ * are device memory; the pointer arrays are host arrays. */
+FCAMD_API long long fcamd_rl_workspace_bytes(int nviews, const int* imDim, const int* kernelDims,
+                                             int fused);
+FCAMD_API int fcamd_rl_fused_available(int nviews, const int* imDim, const int* kernelDims);
+FCAMD_API int fcamd_rl_multiview_device(float* d_psi, const float* const* d_views, int nviews,
+                                        const int* imDim, const float* const* d_kernels,
+                                        const int* kernelDims, int iterations, float eps,
+                                        int init_psi, int fused, void* d_workspace,
+                                        long long workspace_bytes, void* stream);
+
 /* Per-pass HIP-event timing of the convolution passes (diagnostics and the
  * benchmark's roofline figures).  While enabled every pass launch is
  * bracketed by two events on its stream.  fcamd_pass_times waits for them,
