@@ -183,6 +183,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -208,6 +209,11 @@ def main():
 
     def step():
         fc.convolve3d(img, psf, out=out, workspace=ws, stream=stream)
+
+    # plan-time tuning of the kernel variants for this shape (not timed)
+    if not args.no_autotune:
+        fc.autotune(img.shape, psf.shape, reps=5, stream=stream)
+    variants = fc.selected_variants(img.shape)
 
     for _ in range(args.warmup):
         step()
@@ -308,6 +314,7 @@ def main():
             "frac": conv_bytes / (ms_step * 1e-3) / HBM_PEAK,
         },
         "hbm_copy_GBps": copy_GBps,
+        "variants": variants,
         "pass_ms": {k: round(v, 5) for k, v in per_pass.items()},
         "pass_GBps": {k: round(pbytes[k] / (v * 1e-3) / 1e9, 1) for k, v in per_pass.items()},
         "wall_s_timed_region": wall,

@@ -169,6 +169,24 @@ def use_static_kernels(enable: bool) -> None:
     lib().fcamd_use_static_kernels(1 if enable else 0)
 
 
+def autotune(image_shape, kernel_shape, reps=5, stream=None):
+    """Time every compile-time kernel variant for this convolution shape
+    (C-order) on the current device and keep the fastest per pass."""
+    check(lib().fcamd_autotune(ints(tuple(image_shape)[::-1]), ints(tuple(kernel_shape)[::-1]),
+                               int(reps), _stream_handle(stream)))
+
+
+def selected_variants(image_shape):
+    """{pass name: variant name} currently selected for a C-order shape."""
+    nz, ny, nx = image_shape
+    out = {}
+    for p, (name, L) in enumerate([("x_forward", nx), ("x_inverse", nx), ("y", ny), ("z", nz)]):
+        i = lib().fcamd_selected_static_variant(p, L)
+        if i >= 0:
+            out[name] = lib().fcamd_static_variant_name(p, L, i).decode()
+    return out
+
+
 def _stream_handle(stream):
     if stream is None:
         import torch

@@ -61,6 +61,8 @@ SIGNATURES = {
     "fcamd_rl_fused_available": (_i, [_i, _ip, _ip]),
     "fcamd_rl_multiview_device": (_i, [_vp, _vp, _i, _ip, _vp, _ip, _i, ctypes.c_float, _i, _i,
                                        _vp, _ll, _vp]),
+    "fcamd_autotune": (_i, [_ip, _ip, _i, _vp]),
+    "fcamd_autotune_load": (_i, [ctypes.c_char_p]),
     "fcamd_pass_timing": (None, [_i]),
     "fcamd_pass_times": (_i, [ctypes.POINTER(ctypes.c_double), _ip, _i]),
     "fcamd_pass_name": (ctypes.c_char_p, [_i]),

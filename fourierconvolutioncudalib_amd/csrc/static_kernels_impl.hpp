@@ -173,9 +173,9 @@ sb_xinv(const XPassArgs* __restrict__ args, const int2* __restrict__ tasks) {
 // column pass (y): tiles of TX bins x L positions; tile index
 // o * ntx + xt with ntx = ceil(nbins / TX)
 // ---------------------------------------------------------------------------
-template <int L, int TX>
+template <int L, int TX, bool REV = false>
 struct ColTile {
-  using P = PlanOf<L, false>;
+  using P = PlanOf<L, REV>;
   static constexpr int MAXE = P::MAXE();
   const ColArgs* a;
   long base_src, base_dst;
@@ -194,7 +194,7 @@ struct ColTile {
     const int center = a->center, nvalid = a->nvalid;
     const long es = a->src_elem;
     const bool okk = ok;
-    sfft_for_inputs<L, false>(t, [&](int slot, int pos) {
+    sfft_for_inputs<L, REV>(t, [&](int slot, int pos) {
       int j = pos + center;
       if (j >= L) j -= L;
       float2 val = make_float2(0.f, 0.f);
@@ -206,7 +206,7 @@ struct ColTile {
     if (!ok) return;
     float2* dst = a->dst + base_dst;
     const long es = a->dst_elem;
-    sfft_for_outputs<L, false>(t, [&](int slot, int pos) { dst[(long)pos * es] = v[slot]; });
+    sfft_for_outputs<L, REV>(t, [&](int slot, int pos) { dst[(long)pos * es] = v[slot]; });
   }
 };
 
@@ -218,25 +218,26 @@ __device__ __forceinline__ const ColArgs& col2_args(const Col2& a2, long tile, l
   return main ? a2.m : a2.k;
 }
 
-template <int L, int TX, int DIR>
+template <int L, int TX, int DIR, bool REV = false>
 __device__ __forceinline__ void col_body(const ColArgs& args, long local, float2* lds) {
-  using CT = ColTile<L, TX>;
+  using CT = ColTile<L, TX, REV>;
   const int c = threadIdx.x % TX, t = threadIdx.x / TX;
   const CT tile(args, local, c);
   float2 v[CT::MAXE];
   tile.load(t, v);
   auto addr = [&](int pos) { return pos * TX + c; };
-  sfft_run<L, false, DIR>(v, t, lds, addr, args.fft.stw);
+  sfft_run<L, REV, DIR>(v, t, lds, addr, args.fft.stw);
   tile.store(t, v);
 }
 
-template <int L, int TX, int DIR>
+// REV runs the radix plan in the opposite stage order
+template <int L, int TX, int DIR, bool REV>
 __global__ void __launch_bounds__(TX * threads_per_transform<L>())
 sk_col(Col2 a2) {
   __shared__ float2 lds[L * TX];
   long local;
   const ColArgs& a = col2_args(a2, blockIdx.x, &local);
-  col_body<L, TX, DIR>(a, local, lds);
+  col_body<L, TX, DIR, REV>(a, local, lds);
 }
 
 template <int L, int TX, int DIR>
@@ -470,15 +471,16 @@ Col2 col2_tiles(const Col2& a) {
   return b;
 }
 
-template <int L, int WANT>
+template <int L, int WANT, bool REV = false>
 bool sk_launch_col(const Col2& a, int dir, hipStream_t st) {
+  if (REV && PlanOf<L, false>::S < 2) return false;  // one stage: same as forward order
   constexpr int TX = fit_tile<L, WANT>();
   const Col2 b = col2_tiles<L, TX>(a);
   const dim3 block(TX * threads_per_transform<L>());
   if (dir < 0)
-    hipLaunchKernelGGL((sk_col<L, TX, -1>), dim3((unsigned)b.ntiles), block, 0, st, b);
+    hipLaunchKernelGGL((sk_col<L, TX, -1, REV>), dim3((unsigned)b.ntiles), block, 0, st, b);
   else
-    hipLaunchKernelGGL((sk_col<L, TX, +1>), dim3((unsigned)b.ntiles), block, 0, st, b);
+    hipLaunchKernelGGL((sk_col<L, TX, +1, REV>), dim3((unsigned)b.ntiles), block, 0, st, b);
   return true;
 }
 
@@ -560,7 +562,8 @@ StaticEntry make_static_entry() {
   e.xinv = {{"p4", &sk_launch_xinv<L, 4>}, {"p2", &sk_launch_xinv<L, 2>}};
   e.col = {{"t16", &sk_launch_col<L, 16>},
            {"t16-pipe", &sk_launch_col_pipe<L, 16>},
-           {"t8-pipe", &sk_launch_col_pipe<L, 8>}};
+           {"t8-pipe", &sk_launch_col_pipe<L, 8>},
+           {"t16-revradix", &sk_launch_col<L, 16, true>}};
   e.zfused = {{"t16-fft", &sk_launch_zfused_fft<L, 16>}, {"t16-dft", &sk_launch_zfused_dft<L, 16>}};
   e.batch.pairs = fit_pairs<L, 4>();
   e.batch.tx = fit_tile<L, 16>();

@@ -1,0 +1,156 @@
+// tune.cpp -- plan-time autotuner for the compile-time kernel variants.
+//
+// For one convolution shape it times every variant of every pass that has
+// more than one (x-R2C row pairs, x-C2R row pairs, y tile shape / pipelining
+// / radix order, z kernel-spectrum method) with the others fixed, keeps the
+// fastest, and moves to the next pass (coordinate descent over passes).  The
+// choice is per (pass, transform length) and process-wide.  With
+// FCAMD_TUNE_CACHE set, choices are appended to that file as
+// "pass length variant-name milliseconds" lines and re-applied by
+// fcamd_autotune_load() (called on the first tuning request).
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <mutex>
+#include <sstream>
+#include <string>
+
+#include "../../include/fourierconvolution_amd.h"
+#include "conv_driver.hpp"
+#include "static_registry.hpp"
+
+namespace fcamd {
+namespace {
+
+std::mutex g_tune_mu;
+
+void apply_cached(const std::string& path) {
+  std::ifstream in(path);
+  std::string line;
+  while (std::getline(in, line)) {
+    std::istringstream ss(line);
+    int pass = -1, L = 0;
+    std::string name;
+    if (!(ss >> pass >> L >> name)) continue;
+    StaticEntry* e = static_lookup(L);
+    if (!e || pass < 0 || pass >= kSNumPasses) continue;
+    for (int i = 0; i < e->count(pass); ++i)
+      if (name == e->name(pass, i)) e->selected[pass] = i;
+  }
+}
+
+float time_convolution(const ConvShape& s, float* out, const float* img, const float* ker, void* ws,
+                       hipStream_t st, int reps, hipEvent_t e0, hipEvent_t e1) {
+  for (int i = 0; i < 2; ++i) enqueue_convolution(s, out, img, ker, Op::kConvolve, ws, st);
+  FC_HIP_CHECK(hipEventRecord(e0, st));
+  for (int i = 0; i < reps; ++i) enqueue_convolution(s, out, img, ker, Op::kConvolve, ws, st);
+  FC_HIP_CHECK(hipEventRecord(e1, st));
+  FC_HIP_CHECK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  FC_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+  return ms / (float)reps;
+}
+
+template <class F>
+int tune_guarded(F&& f) {
+  clear_error();
+  try {
+    f();
+    return kOk;
+  } catch (const Error& e) {
+    set_error(e.code, e.msg);
+    return e.code;
+  } catch (...) {
+    set_error(kInternal, "unexpected internal error");
+    return kInternal;
+  }
+}
+
+}  // namespace
+}  // namespace fcamd
+
+using namespace fcamd;
+
+extern "C" {
+
+int fcamd_autotune_load(const char* path) {
+  return tune_guarded([&] {
+    const char* p = path ? path : std::getenv("FCAMD_TUNE_CACHE");
+    if (p && p[0]) {
+      std::lock_guard<std::mutex> lk(g_tune_mu);
+      apply_cached(p);
+    }
+  });
+}
+
+int fcamd_autotune(const int* imDim, const int* kernelDim, int reps, void* stream) {
+  static std::once_flag loaded;
+  std::call_once(loaded, [] { (void)fcamd_autotune_load(nullptr); });
+  return tune_guarded([&] {
+    FC_REQUIRE(imDim && kernelDim, kInvalidArgument, "null dimension array");
+    ConvShape s{imDim[0], imDim[1], imDim[2], kernelDim[0], kernelDim[1], kernelDim[2],
+                kernelDim[0] / 2, kernelDim[1] / 2, kernelDim[2] / 2};
+    validate_shape(s);
+    if (reps < 1) reps = 5;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    const size_t nimg = sizeof(float) * (size_t)s.voxels();
+    const size_t nker = sizeof(float) * (size_t)s.kx * s.ky * s.kz;
+    const size_t nws = workspace_layout(s).total();
+    float *img = nullptr, *out = nullptr, *ker = nullptr;
+    void* ws = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    auto cleanup = [&] {
+      if (img) (void)hipFree(img);
+      if (out) (void)hipFree(out);
+      if (ker) (void)hipFree(ker);
+      if (ws) (void)hipFree(ws);
+      if (e0) (void)hipEventDestroy(e0);
+      if (e1) (void)hipEventDestroy(e1);
+    };
+    try {
+      FC_HIP_CHECK(hipMalloc(&img, nimg));
+      FC_HIP_CHECK(hipMalloc(&out, nimg));
+      FC_HIP_CHECK(hipMalloc(&ker, nker));
+      FC_HIP_CHECK(hipMalloc(&ws, nws));
+      // contents do not affect the timing; a fixed byte pattern keeps the
+      // values finite (0x3c3c3c3c ~ 0.0115f)
+      FC_HIP_CHECK(hipMemsetAsync(img, 0x3c, nimg, st));
+      FC_HIP_CHECK(hipMemsetAsync(ker, 0x3c, nker, st));
+      FC_HIP_CHECK(hipEventCreate(&e0));
+      FC_HIP_CHECK(hipEventCreate(&e1));
+      const int axis_len[kSNumPasses] = {s.nx, s.nx, s.ny, s.nz};
+      const char* cache = std::getenv("FCAMD_TUNE_CACHE");
+      for (int pass = 0; pass < kSNumPasses; ++pass) {
+        StaticEntry* e = static_lookup(axis_len[pass]);
+        if (!e || e->count(pass) < 2) continue;
+        int best = e->selected[pass];
+        float best_ms = 1e30f;
+        for (int v = 0; v < e->count(pass); ++v) {
+          e->selected[pass] = v;
+          const float ms = time_convolution(s, out, img, ker, ws, st, reps, e0, e1);
+          if (ms < best_ms) {
+            best_ms = ms;
+            best = v;
+          }
+        }
+        e->selected[pass] = best;
+        if (cache && cache[0]) {
+          if (FILE* f = std::fopen(cache, "a")) {
+            std::fprintf(f, "%d %d %s %.4f\n", pass, axis_len[pass], e->name(pass, best), best_ms);
+            std::fclose(f);
+          }
+        }
+      }
+    } catch (...) {
+      (void)hipStreamSynchronize(st);
+      cleanup();
+      throw;
+    }
+    FC_HIP_CHECK(hipStreamSynchronize(st));
+    cleanup();
+  });
+}
+
+}  // extern "C"

@@ -140,6 +140,16 @@ FCAMD_API const char* fcamd_static_variant_name(int pass, int length, int index)
 FCAMD_API int fcamd_select_static_variant(int pass, int length, int index);
 FCAMD_API int fcamd_selected_static_variant(int pass, int length);
 
+/* Plan-time autotuner: for a convolution of imDim / kernelDim on the current
+ * device, times every variant of every pass (reps timed runs each, on
+ * `stream`, with scratch buffers it allocates and frees) and keeps the
+ * fastest per (pass, length).  Do not run convolutions concurrently on other
+ * threads while it runs.  With FCAMD_TUNE_CACHE set, choices are appended to
+ * that file; fcamd_autotune_load(path or NULL for the variable) re-applies
+ * them. */
+FCAMD_API int fcamd_autotune(const int* imDim, const int* kernelDim, int reps, void* stream);
+FCAMD_API int fcamd_autotune_load(const char* path);
+
 /* enable = 0 routes every pass through the runtime-length kernels (for
  * cross-checks and comparisons); 1 restores the default.  Process-wide. */
 FCAMD_API void fcamd_use_static_kernels(int enable);

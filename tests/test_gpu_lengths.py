@@ -100,3 +100,24 @@ def test_every_variant(fc, oracle, L):
     finally:
         for p in cases:
             lib.fcamd_select_static_variant(p, L, 0)
+
+
+def test_autotune_keeps_results_and_writes_cache(fc, oracle, tmp_path, monkeypatch):
+    """The autotuner times every variant, keeps the fastest, records it in
+    the cache file, and the tuned path still matches the oracle."""
+    from fourierconvolutioncudalib_amd import synth
+    cache = tmp_path / "tune.txt"
+    monkeypatch.setenv("FCAMD_TUNE_CACHE", str(cache))
+    shape, kshape = (64, 96, 128), (9, 7, 5)
+    fc.autotune(shape, kshape, reps=2)
+    sel = fc.selected_variants(shape)
+    assert set(sel) == {"x_forward", "x_inverse", "y", "z"}
+    lines = cache.read_text().splitlines()
+    assert len(lines) == 4 and all(len(l.split()) == 4 for l in lines)
+    img = synth.uniform(shape, 17)
+    k = synth.random_psf(kshape, 18)
+    assert oracle.rel_l2(fc.convolve(img, k), oracle.numpy_fft_conv(img, k)) < 1e-5
+    # restore defaults for the tests that follow
+    lib = fc.lib()
+    for p, L in [(0, 128), (1, 128), (2, 96), (3, 64)]:
+        lib.fcamd_select_static_variant(p, L, 0)
