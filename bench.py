@@ -41,7 +41,7 @@ def pass_bytes(nx, ny, nz, kx, ky, kz):
         "kernel_planes_yfwd": 8 * nb * ky * kz + 8 * nb * ny * kz,
         "image_xfwd": 4 * N + 8 * Nc,
         "image_yfwd": 16 * Nc,
-        "zfused": 16 * Nc,
+        "zfused": 16 * Nc + 8 * nb * ny * kz,
         "image_yinv": 16 * Nc,
         "image_xinv": 8 * Nc + 4 * N,
     }, 8 * N + 64 * Nc + 4 * kx * ky * kz

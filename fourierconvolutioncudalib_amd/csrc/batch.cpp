@@ -9,9 +9,17 @@
 // run on forked streams; the pass boundary joins them.  The whole sequence
 // can be captured once into a hipGraph and replayed.
 //
+// Blocks are executed in waves: consecutive blocks whose workspaces add up
+// to at most a byte budget form one wave, every wave runs all passes before
+// the next starts, and all waves share one workspace region.  With a budget
+// below the device's last-level (Infinity) cache, a wave's spectra stay in
+// that cache from pass to pass instead of going through HBM, and the
+// workspace shrinks to the largest wave.
+//
 // Blocks whose length has no compile-time kernel fall back to the
 // runtime-length kernels, one launch per block.
 #include <algorithm>
+#include <cstdlib>
 #include <map>
 #include <memory>
 #include <vector>
@@ -25,6 +33,9 @@ namespace fcamd {
 namespace {
 
 size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+
+// Default wave budget (bytes of block workspace per wave; 0 = one wave).
+constexpr size_t kDefaultWaveBytes = 0;
 
 struct Block {
   ConvShape s;
@@ -70,19 +81,68 @@ void add_tasks(std::vector<int2>& tasks, int arg, long count) {
 
 }  // namespace
 
+// Wave partition of n blocks with workspace sizes ws[b]: consecutive blocks
+// while the running sum stays within `budget` (0 = one wave); returns the
+// first block of every wave and the largest wave's workspace.
+std::vector<int> wave_starts(const std::vector<size_t>& ws, size_t budget, size_t* largest) {
+  std::vector<int> starts;
+  size_t cur = 0, big = 0;
+  for (int b = 0; b < (int)ws.size(); ++b) {
+    if (b == 0 || (budget > 0 && cur + ws[b] > budget)) {
+      starts.push_back(b);
+      cur = 0;
+    }
+    cur += ws[b];
+    big = std::max(big, cur);
+  }
+  if (largest) *largest = big;
+  return starts;
+}
+
+// Wave byte budget: FCAMD_BATCH_WAVE_MB when set (0 = one wave), else the
+// default below.
+size_t default_wave_budget() {
+  if (const char* v = std::getenv("FCAMD_BATCH_WAVE_MB")) return (size_t)std::atol(v) << 20;
+  return kDefaultWaveBytes;
+}
+
+size_t block_workspace(const ConvShape& s) { return align256(workspace_layout(s).total()); }
+
+ConvShape block_shape(const int* imDims, const int* kernelDims, int b) {
+  ConvShape s{};
+  s.nx = imDims[3 * b];
+  s.ny = imDims[3 * b + 1];
+  s.nz = imDims[3 * b + 2];
+  s.kx = kernelDims[3 * b];
+  s.ky = kernelDims[3 * b + 1];
+  s.kz = kernelDims[3 * b + 2];
+  s.cx = s.kx / 2;
+  s.cy = s.ky / 2;
+  s.cz = s.kz / 2;
+  return s;
+}
+
 class BatchPlan {
  public:
   BatchPlan(int n, const int* imDims, const int* kernelDims, float* const* outs,
-            const float* const* imgs, const float* const* kers, int op, void* ws, size_t ws_bytes);
+            const float* const* imgs, const float* const* kers, int op, void* ws, size_t ws_bytes,
+            size_t wave_budget);
   ~BatchPlan();
   void enqueue(hipStream_t st);
   void graph_launch(hipStream_t st);
   size_t workspace() const { return ws_need_; }
 
  private:
+  struct Wave {
+    std::vector<Group<XPassArgs>> xf, xi;
+    std::vector<Group<ColArgs>> yf, yi;
+    std::vector<Group<ZArgs>> z;
+    std::vector<int> generic_x, generic_y, generic_z;  // block indices
+  };
   template <class F>
   void fork(hipStream_t st, size_t ngroups, F launch_group);
-  void build_groups();
+  void build_wave(Wave& w, int first, int last);
+  void enqueue_wave(const Wave& w, hipStream_t st);
 
   int dev_ = 0;
   bool conj_ = false;
@@ -90,11 +150,7 @@ class BatchPlan {
   size_t ws_need_ = 0;
   void* ws_ = nullptr;
   bool own_ws_ = false;
-  // per pass
-  std::vector<Group<XPassArgs>> xf_, xi_;
-  std::vector<Group<ColArgs>> yf_, yi_;
-  std::vector<Group<ZArgs>> z_;
-  std::vector<int> generic_x_, generic_y_, generic_z_;  // block indices
+  std::vector<Wave> waves_;
   std::vector<hipStream_t> side_;
   std::vector<hipEvent_t> ev_;
   hipGraphExec_t exec_ = nullptr;
@@ -104,35 +160,25 @@ class BatchPlan {
 
 BatchPlan::BatchPlan(int n, const int* imDims, const int* kernelDims, float* const* outs,
                      const float* const* imgs, const float* const* kers, int op, void* ws,
-                     size_t ws_bytes) {
+                     size_t ws_bytes, size_t wave_budget) {
   FC_REQUIRE(n >= 1, kInvalidArgument, "empty batch");
   FC_REQUIRE(imDims && kernelDims && outs && imgs && kers, kInvalidArgument, "null array");
   FC_REQUIRE(op == FCAMD_CONVOLVE || op == FCAMD_CORRELATE, kInvalidArgument, "unknown op");
   conj_ = op == FCAMD_CORRELATE;
   FC_HIP_CHECK(hipGetDevice(&dev_));
-  size_t off = 0;
-  std::vector<size_t> offs;
+  std::vector<size_t> sizes;
   for (int b = 0; b < n; ++b) {
     Block bl{};
-    bl.s.nx = imDims[3 * b];
-    bl.s.ny = imDims[3 * b + 1];
-    bl.s.nz = imDims[3 * b + 2];
-    bl.s.kx = kernelDims[3 * b];
-    bl.s.ky = kernelDims[3 * b + 1];
-    bl.s.kz = kernelDims[3 * b + 2];
-    bl.s.cx = bl.s.kx / 2;
-    bl.s.cy = bl.s.ky / 2;
-    bl.s.cz = bl.s.kz / 2;
+    bl.s = block_shape(imDims, kernelDims, b);
     validate_shape(bl.s);
     FC_REQUIRE(outs[b] && imgs[b] && kers[b], kInvalidArgument, "null block pointer");
     bl.out = outs[b];
     bl.img = imgs[b];
     bl.ker = kers[b];
-    offs.push_back(off);
-    off += align256(workspace_layout(bl.s).total());
+    sizes.push_back(block_workspace(bl.s));
     blocks_.push_back(bl);
   }
-  ws_need_ = off;
+  const std::vector<int> starts = wave_starts(sizes, wave_budget, &ws_need_);
   if (ws) {
     FC_REQUIRE(ws_bytes >= ws_need_, kInvalidArgument, "batch workspace too small");
     ws_ = ws;
@@ -140,19 +186,20 @@ BatchPlan::BatchPlan(int n, const int* imDims, const int* kernelDims, float* con
     FC_HIP_CHECK(hipMalloc(&ws_, ws_need_));
     own_ws_ = true;
   }
-  for (int b = 0; b < n; ++b) {
-    const WorkspaceLayout wl = workspace_layout(blocks_[b].s);
-    char* base = static_cast<char*>(ws_) + offs[b];
-    blocks_[b].S = reinterpret_cast<float2*>(base);
-    blocks_[b].Kx = reinterpret_cast<float2*>(base + wl.spectrum);
-    blocks_[b].Kxy = reinterpret_cast<float2*>(base + wl.spectrum + wl.krows);
+  // every wave lays its blocks out from the start of the workspace
+  for (size_t w = 0; w < starts.size(); ++w) {
+    const int last = w + 1 < starts.size() ? starts[w + 1] : n;
+    size_t off = 0;
+    for (int b = starts[w]; b < last; ++b) {
+      const WorkspaceLayout wl = workspace_layout(blocks_[b].s);
+      char* base = static_cast<char*>(ws_) + off;
+      blocks_[b].S = reinterpret_cast<float2*>(base);
+      blocks_[b].Kx = reinterpret_cast<float2*>(base + wl.spectrum);
+      blocks_[b].Kxy = reinterpret_cast<float2*>(base + wl.spectrum + wl.krows);
+      off += sizes[b];
+    }
   }
-  build_groups();
-}
-
-void BatchPlan::build_groups() {
   driver_init();
-  std::map<int, std::vector<int>> by_x, by_y, by_z;
   // create every twiddle table now: nothing may allocate or copy while a
   // graph is being captured
   for (const Block& bl : blocks_) {
@@ -160,7 +207,27 @@ void BatchPlan::build_groups() {
     (void)fft_desc(bl.s.ny);
     (void)fft_desc(bl.s.nz);
   }
-  for (int b = 0; b < (int)blocks_.size(); ++b) {
+  waves_.resize(starts.size());
+  size_t nside = 1;
+  for (size_t w = 0; w < starts.size(); ++w) {
+    build_wave(waves_[w], starts[w], w + 1 < starts.size() ? starts[w + 1] : n);
+    nside = std::max({nside, waves_[w].xf.size(), waves_[w].yf.size(), waves_[w].z.size()});
+  }
+  for (size_t i = 0; i < nside; ++i) {
+    hipStream_t st = nullptr;
+    FC_HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    side_.push_back(st);
+  }
+  for (size_t i = 0; i < nside + 1; ++i) {
+    hipEvent_t e = nullptr;
+    FC_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    ev_.push_back(e);
+  }
+}
+
+void BatchPlan::build_wave(Wave& w, int first, int last) {
+  std::map<int, std::vector<int>> by_x, by_y, by_z;
+  for (int b = first; b < last; ++b) {
     by_x[blocks_[b].s.nx].push_back(b);
     by_y[blocks_[b].s.ny].push_back(b);
     by_z[blocks_[b].s.nz].push_back(b);
@@ -169,7 +236,7 @@ void BatchPlan::build_groups() {
   for (auto& kv : by_x) {
     StaticEntry* e = static_lookup(kv.first);
     if (!e || !e->batch.xfwd) {
-      generic_x_.insert(generic_x_.end(), kv.second.begin(), kv.second.end());
+      w.generic_x.insert(w.generic_x.end(), kv.second.begin(), kv.second.end());
       continue;
     }
     const FftDesc fx = fft_desc(kv.first);
@@ -216,14 +283,14 @@ void BatchPlan::build_groups() {
     }
     upload(gf);
     upload(gi);
-    xf_.push_back(gf);
-    xi_.push_back(gi);
+    w.xf.push_back(gf);
+    w.xi.push_back(gi);
   }
   // y forward (+ kernel planes) / inverse
   for (auto& kv : by_y) {
     StaticEntry* e = static_lookup(kv.first);
     if (!e || !e->batch.col) {
-      generic_y_.insert(generic_y_.end(), kv.second.begin(), kv.second.end());
+      w.generic_y.insert(w.generic_y.end(), kv.second.begin(), kv.second.end());
       continue;
     }
     const FftDesc fy = fft_desc(kv.first);
@@ -235,7 +302,6 @@ void BatchPlan::build_groups() {
       const Block& bl = blocks_[b];
       const long nxp = padded_bins(bl.s.nx);
       const int nb = half_bins(bl.s.nx);
-      const long ntx = (nb + TX - 1) / TX;
       ColArgs m{};
       m.fft = fy;
       m.nbins = nb;
@@ -245,29 +311,31 @@ void BatchPlan::build_groups() {
       m.src_elem = m.dst_elem = nxp;
       m.nvalid = bl.s.ny;
       m.center = 0;
+      m.nouter = bl.s.nz;
       ColArgs k = m;
       k.src = bl.Kx;
       k.dst = bl.Kxy;
       k.src_outer = (long)bl.s.ky * nxp;
       k.nvalid = bl.s.ky;
       k.center = bl.s.cy;
+      k.nouter = bl.s.kz;
       gf.args.push_back(k);
-      add_tasks(gf.tasks, (int)gf.args.size() - 1, ntx * bl.s.kz);
+      add_tasks(gf.tasks, (int)gf.args.size() - 1, column_tiles(nb, TX, bl.s.kz));
       gf.args.push_back(m);
-      add_tasks(gf.tasks, (int)gf.args.size() - 1, ntx * bl.s.nz);
+      add_tasks(gf.tasks, (int)gf.args.size() - 1, column_tiles(nb, TX, bl.s.nz));
       gi.args.push_back(m);
-      add_tasks(gi.tasks, (int)gi.args.size() - 1, ntx * bl.s.nz);
+      add_tasks(gi.tasks, (int)gi.args.size() - 1, column_tiles(nb, TX, bl.s.nz));
     }
     upload(gf);
     upload(gi);
-    yf_.push_back(gf);
-    yi_.push_back(gi);
+    w.yf.push_back(gf);
+    w.yi.push_back(gi);
   }
   // fused z
   for (auto& kv : by_z) {
     StaticEntry* e = static_lookup(kv.first);
     if (!e || !e->batch.zfused) {
-      generic_z_.insert(generic_z_.end(), kv.second.begin(), kv.second.end());
+      w.generic_z.insert(w.generic_z.end(), kv.second.begin(), kv.second.end());
       continue;
     }
     const FftDesc fz = fft_desc(kv.first);
@@ -291,33 +359,25 @@ void BatchPlan::build_groups() {
       a.center = bl.s.cz;
       a.conj = conj_ ? 1 : 0;
       a.scale = (float)(1.0 / (double)bl.s.voxels());
+      a.ny = bl.s.ny;
       g.args.push_back(a);
-      add_tasks(g.tasks, (int)g.args.size() - 1, (long)((a.nbins + TX - 1) / TX) * bl.s.ny);
+      add_tasks(g.tasks, (int)g.args.size() - 1, column_tiles(a.nbins, TX, bl.s.ny));
     }
     upload(g);
-    z_.push_back(g);
-  }
-  size_t nside = std::max({xf_.size(), yf_.size(), z_.size(), (size_t)1});
-  for (size_t i = 0; i < nside; ++i) {
-    hipStream_t s = nullptr;
-    FC_HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-    side_.push_back(s);
-  }
-  for (size_t i = 0; i < nside + 1; ++i) {
-    hipEvent_t e = nullptr;
-    FC_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    ev_.push_back(e);
+    w.z.push_back(g);
   }
 }
 
 BatchPlan::~BatchPlan() {
   if (exec_) (void)hipGraphExecDestroy(exec_);
   if (graph_) (void)hipGraphDestroy(graph_);
-  for (auto& g : xf_) release(g);
-  for (auto& g : xi_) release(g);
-  for (auto& g : yf_) release(g);
-  for (auto& g : yi_) release(g);
-  for (auto& g : z_) release(g);
+  for (Wave& w : waves_) {
+    for (auto& g : w.xf) release(g);
+    for (auto& g : w.xi) release(g);
+    for (auto& g : w.yf) release(g);
+    for (auto& g : w.yi) release(g);
+    for (auto& g : w.z) release(g);
+  }
   for (auto s : side_) (void)hipStreamDestroy(s);
   if (cap_) (void)hipStreamDestroy(cap_);
   for (auto e : ev_) (void)hipEventDestroy(e);
@@ -340,43 +400,49 @@ void BatchPlan::fork(hipStream_t st, size_t ngroups, F launch_group) {
   }
 }
 
-void BatchPlan::enqueue(hipStream_t st) {
+void BatchPlan::enqueue_wave(const Wave& w, hipStream_t st) {
   // x forward (image and kernel rows)
-  fork(st, xf_.size(), [&](size_t i, hipStream_t s) {
-    auto& g = xf_[i];
+  fork(st, w.xf.size(), [&](size_t i, hipStream_t s) {
+    auto& g = w.xf[i];
     g.e->batch.xfwd(g.d_args, g.d_tasks, (int)g.tasks.size(), s);
   });
-  for (int b : generic_x_) {
+  for (int b : w.generic_x) {
     const Block& bl = blocks_[b];
     pass_x_forward(bl.s, bl.img, bl.S, bl.ker, bl.Kx, st);
   }
-  // y forward (image and kernel planes); blocks already done above skip
-  fork(st, yf_.size(), [&](size_t i, hipStream_t s) {
-    auto& g = yf_[i];
+  // y forward (image and kernel planes)
+  fork(st, w.yf.size(), [&](size_t i, hipStream_t s) {
+    auto& g = w.yf[i];
     g.e->batch.col(g.d_args, g.d_tasks, (int)g.tasks.size(), -1, s);
   });
-  for (int b : generic_y_) {
+  for (int b : w.generic_y) {
     const Block& bl = blocks_[b];
     pass_y_forward(bl.s, bl.S, bl.Kx, bl.Kxy, st);
   }
-  fork(st, z_.size(), [&](size_t i, hipStream_t s) {
-    auto& g = z_[i];
+  fork(st, w.z.size(), [&](size_t i, hipStream_t s) {
+    auto& g = w.z[i];
     g.e->batch.zfused(g.d_args, g.d_tasks, (int)g.tasks.size(), s);
   });
-  for (int b : generic_z_) {
+  for (int b : w.generic_z) {
     const Block& bl = blocks_[b];
     z_pass(bl.s, bl.S, bl.Kxy, false, conj_, st);
   }
-  fork(st, yi_.size(), [&](size_t i, hipStream_t s) {
-    auto& g = yi_[i];
+  fork(st, w.yi.size(), [&](size_t i, hipStream_t s) {
+    auto& g = w.yi[i];
     g.e->batch.col(g.d_args, g.d_tasks, (int)g.tasks.size(), +1, s);
   });
-  for (int b : generic_y_) pass_y_inverse(blocks_[b].s, blocks_[b].S, st);
-  fork(st, xi_.size(), [&](size_t i, hipStream_t s) {
-    auto& g = xi_[i];
+  for (int b : w.generic_y) pass_y_inverse(blocks_[b].s, blocks_[b].S, st);
+  fork(st, w.xi.size(), [&](size_t i, hipStream_t s) {
+    auto& g = w.xi[i];
     g.e->batch.xinv(g.d_args, g.d_tasks, (int)g.tasks.size(), s);
   });
-  for (int b : generic_x_) pass_x_inverse(blocks_[b].s, blocks_[b].S, blocks_[b].out, st);
+  for (int b : w.generic_x) pass_x_inverse(blocks_[b].s, blocks_[b].S, blocks_[b].out, st);
+}
+
+void BatchPlan::enqueue(hipStream_t st) {
+  // waves share the workspace: stream order on `st` (every fork joins back
+  // into it) keeps wave w+1 from starting before wave w has finished
+  for (const Wave& w : waves_) enqueue_wave(w, st);
   FC_HIP_CHECK(hipGetLastError());
 }
 
@@ -433,15 +499,15 @@ long long fcamd_batch_workspace_bytes(int nblocks, const int* imDims, const int*
   long long r = -1;
   (void)batch_guarded([&] {
     FC_REQUIRE(nblocks >= 1 && imDims && kernelDims, kInvalidArgument, "bad batch description");
-    size_t tot = 0;
+    std::vector<size_t> sizes;
     for (int b = 0; b < nblocks; ++b) {
-      ConvShape s{imDims[3 * b], imDims[3 * b + 1], imDims[3 * b + 2],
-                  kernelDims[3 * b], kernelDims[3 * b + 1], kernelDims[3 * b + 2],
-                  kernelDims[3 * b] / 2, kernelDims[3 * b + 1] / 2, kernelDims[3 * b + 2] / 2};
+      const ConvShape s = block_shape(imDims, kernelDims, b);
       validate_shape(s);
-      tot += align256(workspace_layout(s).total());
+      sizes.push_back(block_workspace(s));
     }
-    r = (long long)tot;
+    size_t largest = 0;
+    (void)wave_starts(sizes, default_wave_budget(), &largest);
+    r = (long long)largest;
   });
   return r;
 }
@@ -452,7 +518,7 @@ void* fcamd_batch_create(int nblocks, const int* imDims, const int* kernelDims, 
   BatchPlan* p = nullptr;
   (void)batch_guarded([&] {
     p = new BatchPlan(nblocks, imDims, kernelDims, d_outs, d_images, d_kernels, op, d_workspace,
-                      (size_t)(workspace_bytes < 0 ? 0 : workspace_bytes));
+                      (size_t)(workspace_bytes < 0 ? 0 : workspace_bytes), default_wave_budget());
   });
   return p;
 }

@@ -49,8 +49,113 @@ constexpr float kS73 = 0.43388373911755812048f;  // sin(6pi/7)
 constexpr float kR2 = 0.70710678118654752440f;   // sqrt(1/2)
 }  // namespace k
 
+// ---------------------------------------------------------------------------
+// compile-time sin/cos of 2*pi*m/n in double precision (range reduction by
+// symmetry to [0, pi/4], then Taylor series), rounded once to float
+// ---------------------------------------------------------------------------
+namespace cx {
+constexpr double kPi = 3.14159265358979323846264338327950288;
+constexpr double sin_series(double x) {
+  double term = x, sum = x;
+  for (int k = 1; k < 14; ++k) {
+    term *= -x * x / ((2.0 * k) * (2.0 * k + 1.0));
+    sum += term;
+  }
+  return sum;
+}
+constexpr double cos_series(double x) {
+  double term = 1.0, sum = 1.0;
+  for (int k = 1; k < 14; ++k) {
+    term *= -x * x / ((2.0 * k - 1.0) * (2.0 * k));
+    sum += term;
+  }
+  return sum;
+}
+// cos and sin of 2*pi*m/n for integer m, n > 0, exact symmetry handling:
+// the angle is reduced in units of n/8 so the series only sees [0, pi/4].
+struct CS {
+  double c, s;
+};
+constexpr CS cossin_2pi(long m, long n) {
+  m %= n;
+  if (m < 0) m += n;
+  // octant o = floor(8 m / n), remainder r in [0, n/8) as a fraction
+  const long m8 = 8 * m;
+  const long o = m8 / n;
+  const double frac = (double)(m8 - o * n) / (double)n;  // in [0, 1)
+  const double x = frac * (kPi / 4.0);                   // [0, pi/4)
+  const double y = (kPi / 4.0) - x;                      // (0, pi/4]
+  double c = 0.0, s = 0.0;
+  switch (o) {
+    case 0: c = cos_series(x); s = sin_series(x); break;
+    case 1: c = sin_series(y); s = cos_series(y); break;
+    case 2: c = -sin_series(x); s = cos_series(x); break;
+    case 3: c = -cos_series(y); s = sin_series(y); break;
+    case 4: c = -cos_series(x); s = -sin_series(x); break;
+    case 5: c = -sin_series(y); s = -cos_series(y); break;
+    case 6: c = sin_series(x); s = -cos_series(x); break;
+    default: c = cos_series(y); s = -sin_series(y); break;
+  }
+  return CS{c, s};
+}
+template <int R>
+struct Roots {
+  float c[R];
+  float s[R];
+  constexpr Roots() : c(), s() {
+    for (int m = 0; m < R; ++m) {
+      const CS v = cossin_2pi(m, R);
+      c[m] = (float)v.c;
+      s[m] = (float)v.s;
+    }
+  }
+};
+constexpr int smallest_factor(int r) {
+  for (int f : {4, 2, 3, 5, 7})
+    if (r % f == 0 && r != f) return f;
+  return r;
+}
+}  // namespace cx
+
+// Composite radix R = R1 * R2 by an in-register Cooley-Tukey split:
+// y[k1 + R1 k2] = sum_n2 W_R^(n2 k1) W_R2^(n2 k2) sum_n1 x[n1 R2 + n2] W_R1^(n1 k1).
+// Radices with a dedicated butterfly below specialise this template.
 template <int R, int DIR>
-struct Dft;
+struct Dft {
+  static constexpr int R1 = cx::smallest_factor(R);
+  static constexpr int R2 = R / R1;
+  static_assert(R1 > 1 && R2 > 1, "composite radix expected");
+  __device__ __forceinline__ static void run(float2* v) {
+    constexpr cx::Roots<R> W{};
+    float2 t[R];
+#pragma unroll
+    for (int n2 = 0; n2 < R2; ++n2) {
+      float2 a[R1];
+#pragma unroll
+      for (int n1 = 0; n1 < R1; ++n1) a[n1] = v[n1 * R2 + n2];
+      Dft<R1, DIR>::run(a);
+#pragma unroll
+      for (int k1 = 0; k1 < R1; ++k1) {
+        const int m = (n2 * k1) % R;
+        if (m == 0) {
+          t[k1 * R2 + n2] = a[k1];
+        } else {
+          const float2 w = make_float2(W.c[m], DIR * W.s[m]);
+          t[k1 * R2 + n2] = cmul(a[k1], w);
+        }
+      }
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) {
+      float2 b[R2];
+#pragma unroll
+      for (int n2 = 0; n2 < R2; ++n2) b[n2] = t[k1 * R2 + n2];
+      Dft<R2, DIR>::run(b);
+#pragma unroll
+      for (int k2 = 0; k2 < R2; ++k2) v[k1 + R1 * k2] = b[k2];
+    }
+  }
+};
 
 template <int DIR>
 struct Dft<1, DIR> {

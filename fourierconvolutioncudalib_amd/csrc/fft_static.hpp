@@ -1,7 +1,7 @@
 // fft_static.hpp -- compile-time-length Stockham FFT held in registers.
 //
-// A transform of length L is served by T = L / Rmax threads (Rmax = largest
-// radix of the plan).  In stage s (radix R, Ns = product of the earlier
+// A transform of length L is served by T threads (L / Rmax for the classic
+// plan, Rmax = largest radix; chosen by a cost model for composite plans).  In stage s (radix R, Ns = product of the earlier
 // radices, M = L / R) thread t owns butterflies b = t + i*T (i < NB,
 // predicated b < M); butterfly b reads positions b + r*M and writes
 // positions (b / Ns)*Ns*R + b % Ns + r*Ns (Stockham auto-sort, so the result
@@ -23,12 +23,14 @@ namespace fcamd {
 struct StaticPlanData {
   int n = 0;
   int r[16] = {};
+  int t = 0;  // threads per transform
 };
 
-// Radix plan for a 7-smooth length: the power-of-two part in the fewest
+// Classic plan for a 7-smooth length: the power-of-two part in the fewest
 // stages of radix <= 16 with the bits spread evenly (256 = 16*16,
-// 512 = 8*8*8, 1024 = 16*8*8, 128 = 16*8), then 3s, 5s and 7s.
-constexpr StaticPlanData static_plan(int L) {
+// 512 = 8*8*8, 1024 = 16*8*8, 128 = 16*8), then 3s, 5s and 7s; one thread
+// per largest-radix butterfly.
+constexpr StaticPlanData classic_plan(int L) {
   StaticPlanData p{};
   int a = 0;
   int n = L;
@@ -41,8 +43,107 @@ constexpr StaticPlanData static_plan(int L) {
   while (n % 3 == 0) { p.r[p.n++] = 3; n /= 3; }
   while (n % 5 == 0) { p.r[p.n++] = 5; n /= 5; }
   while (n % 7 == 0) { p.r[p.n++] = 7; n /= 7; }
-  if (n != 1) p.n = -1;  // not 7-smooth: no static plan
+  if (n != 1) {
+    p.n = -1;  // not 7-smooth: no static plan
+    return p;
+  }
+  int rmax = 1;
+  for (int i = 0; i < p.n; ++i) rmax = p.r[i] > rmax ? p.r[i] : rmax;
+  p.t = L / rmax;
   return p;
+}
+
+// Composite plan: the fewest stages with radices from {2..16} including
+// the composite butterflies 6, 9, 10, 12, 14, 15 (240 = 16*15,
+// 210 = 15*14, 500 = 10*10*5), and the thread count per transform T (one of
+// the stages' butterfly counts M = L / R) chosen with the radix multiset to
+// minimise a per-lane cost model: each stage costs T * ceil(M / T)
+// butterflies (idle lanes included) of an estimated flop count plus
+// twiddle and LDS-exchange work.  At most 32 values per thread are held in
+// registers.
+namespace plan_detail {
+constexpr int kRadix[13] = {16, 15, 14, 12, 10, 9, 8, 7, 6, 5, 4, 3, 2};
+constexpr int butterfly_flops(int r) {
+  switch (r) {
+    case 2: return 4;
+    case 3: return 16;
+    case 4: return 16;
+    case 5: return 40;
+    case 6: return 56;
+    case 7: return 72;
+    case 8: return 52;
+    case 9: return 120;
+    case 10: return 124;
+    case 12: return 148;
+    case 14: return 208;
+    case 15: return 248;
+    case 16: return 144;
+    default: return 1 << 20;
+  }
+}
+struct Best {
+  StaticPlanData plan{};
+  long cost = -1;
+};
+constexpr void consider(const StaticPlanData& cur, int L, Best& best) {
+  // candidate thread counts: one butterfly per thread in some stage
+  for (int c = 0; c < cur.n; ++c) {
+    const int T = L / cur.r[c];
+    if (T > 256) continue;
+    long cost = 0;
+    bool ok = true;
+    for (int s = 0; s < cur.n; ++s) {
+      const int R = cur.r[s], M = L / R, nb = (M + T - 1) / T;
+      if (nb * R > 32) ok = false;
+      const int w = butterfly_flops(R) + (s > 0 ? 6 * (R - 1) + 4 * R : 0);
+      cost += (long)T * nb * w;
+    }
+    if (ok && (best.cost < 0 || cost < best.cost)) {
+      best.cost = cost;
+      best.plan = cur;
+      best.plan.t = T;
+    }
+  }
+}
+constexpr void search(int rem, int first, StaticPlanData& cur, int S, int L, Best& best) {
+  if (cur.n == S) {
+    if (rem == 1) consider(cur, L, best);
+    return;
+  }
+  for (int i = first; i < 13; ++i) {
+    const int R = kRadix[i];
+    if (rem % R != 0) continue;
+    cur.r[cur.n++] = R;
+    search(rem / R, i, cur, S, L, best);
+    --cur.n;
+  }
+}
+}  // namespace plan_detail
+
+constexpr StaticPlanData composite_plan(int L) {
+  const StaticPlanData c = classic_plan(L);
+  if (c.n <= 0) return c;
+  for (int S = 1; S <= c.n; ++S) {
+    plan_detail::Best best{};
+    StaticPlanData cur{};
+    plan_detail::search(L, 0, cur, S, L, best);
+    if (best.cost >= 0) return best.plan;
+  }
+  return c;
+}
+
+#ifndef FC_PLAN_COMPOSITE
+#define FC_PLAN_COMPOSITE 0
+#endif
+
+// Plan family used for length L: the composite plan where it needs fewer
+// stages (one LDS exchange less per stage saved) than the classic plan.
+constexpr bool uses_composite_plan(int L) {
+  return FC_PLAN_COMPOSITE != 0 && composite_plan(L).n < classic_plan(L).n;
+}
+
+constexpr StaticPlanData static_plan(int L) {
+  return uses_composite_plan(L) ? composite_plan(L) : classic_plan(L);
 }
 
 // Per-stage twiddle tables of a static plan, stored for the forward stage
@@ -73,7 +174,7 @@ struct PlanOf {
   static_assert(data.n > 0, "length without a static plan");
   static constexpr int S = data.n;
   static constexpr int RMAX = plan_rmax(data);
-  static constexpr int T = L / RMAX;
+  static constexpr int T = data.t;
   static constexpr int R(int s) { return REV ? data.r[S - 1 - s] : data.r[s]; }
   static constexpr int Ns(int s) {
     int v = 1;

@@ -35,6 +35,7 @@ struct ColArgs {
   long dst_outer, dst_elem;
   int nvalid;  // source positions present (column length, or ky)
   int center;  // circular placement offset (0, or ky / 2)
+  long nouter;  // tile rows (grid rows): nz for the spectrum, kz for kernel planes
 };
 
 // fused z pass: s (in place) <- IFFTz( FFTz(s) * op(FFTz(k)) * scale )
@@ -50,7 +51,40 @@ struct ZArgs {
   int center;           // kz / 2, or 0
   int conj;             // use conj(FFTz(k)) (adjoint / correlation)
   float scale;          // 1 / (nx*ny*nz)
+  long ny;              // tile rows
 };
+
+// Column tiles of TX consecutive bins over `nouter` rows.  When the bins end
+// in a single trailing bin (even nx: bin nx/2 after full tiles), that bin is
+// not given a nearly empty tile per row; instead "Nyquist tiles" gather it
+// from TX consecutive rows, lane c taking row j*TX + c.
+__host__ __device__ inline long column_tiles(int nbins, int TX, long nouter) {
+  const int nfull = nbins / TX, rem = nbins - nfull * TX;
+  if (rem == 1 && nfull > 0) return (long)nfull * nouter + (nouter + TX - 1) / TX;
+  return (long)((nbins + TX - 1) / TX) * nouter;
+}
+
+__host__ __device__ inline void column_tile_decode(int nbins, int TX, long nouter, long tile, int c,
+                                                   long* o, int* x, bool* ok) {
+  const int nfull = nbins / TX, rem = nbins - nfull * TX;
+  if (rem == 1 && nfull > 0) {
+    const long full = (long)nfull * nouter;
+    if (tile < full) {
+      *o = tile / nfull;
+      *x = (int)(tile - *o * nfull) * TX + c;
+      *ok = true;
+    } else {
+      *o = (tile - full) * TX + c;
+      *x = nbins - 1;
+      *ok = *o < nouter;
+    }
+    return;
+  }
+  const int ntx = (nbins + TX - 1) / TX;
+  *o = tile / ntx;
+  *x = (int)(tile - *o * ntx) * TX + c;
+  *ok = *x < nbins;
+}
 
 // A pass launch that also carries the compact kernel's rows / planes as
 // extra workgroups (they share the transform length with the image).
@@ -80,6 +114,7 @@ struct RLZArgs {
   long s_row, s_plane;  // strides of y and z (all spectra share the layout)
   int nviews;
   float scale;                       // 1 / N
+  long ny;                           // tile rows
   const float2* src;                 // fan-out input (FFTxy of psi)
   float2* dst;                       // sum output
   float2* view[kMaxViews];           // per-view spectra (fan-out out / sum in)

@@ -37,11 +37,11 @@ sk_zfan(RLZArgs a) {
   using P = PlanOf<L, false>;
   __shared__ float2 lds[L * TX];
   const int c = threadIdx.x % TX, t = threadIdx.x / TX;
-  const int ntx = (a.nbins + TX - 1) / TX;
-  const long y = blockIdx.x / ntx;
-  const int x = (int)(blockIdx.x - y * ntx) * TX + c;
-  const bool ok = x < a.nbins;
-  const long base = y * a.s_row + x;
+  long y;
+  int x;
+  bool ok;
+  column_tile_decode(a.nbins, TX, a.ny, blockIdx.x, c, &y, &x, &ok);
+  const long base = ok ? y * a.s_row + x : 0;
   auto addr = [&](int pos) { return pos * TX + c; };
   float2 X[P::MAXE()];
   sfft_for_inputs<L, false>(t, [&](int slot, int pos) {
@@ -67,11 +67,11 @@ sk_zsum(RLZArgs a) {
   using P = PlanOf<L, false>;
   __shared__ float2 lds[L * TX];
   const int c = threadIdx.x % TX, t = threadIdx.x / TX;
-  const int ntx = (a.nbins + TX - 1) / TX;
-  const long y = blockIdx.x / ntx;
-  const int x = (int)(blockIdx.x - y * ntx) * TX + c;
-  const bool ok = x < a.nbins;
-  const long base = y * a.s_row + x;
+  long y;
+  int x;
+  bool ok;
+  column_tile_decode(a.nbins, TX, a.ny, blockIdx.x, c, &y, &x, &ok);
+  const long base = ok ? y * a.s_row + x : 0;
   auto addr = [&](int pos) { return pos * TX + c; };
   float2 acc[P::MAXE()];
 #pragma unroll
@@ -197,22 +197,24 @@ sk_xupdate(RLXArgs a) {
 template <int L>
 void rl_launch_zfan(const RLZArgs& a, long ny, hipStream_t st) {
   constexpr int TX = fit_tile<L, 16>();
-  const long ntx = (a.nbins + TX - 1) / TX;
-  hipLaunchKernelGGL((sk_zfan<L, TX>), dim3((unsigned)(ntx * ny)), dim3(TX * threads_per_transform<L>()),
-                     0, st, a);
+  RLZArgs b = a;
+  b.ny = ny;
+  hipLaunchKernelGGL((sk_zfan<L, TX>), dim3((unsigned)column_tiles(a.nbins, TX, ny)),
+                     dim3(TX * threads_per_transform<L>()), 0, st, b);
 }
 
 template <int L>
 void rl_launch_zsum(const RLZArgs& a, long ny, hipStream_t st) {
   constexpr int TX = fit_tile<L, 16>();
-  const long ntx = (a.nbins + TX - 1) / TX;
-  hipLaunchKernelGGL((sk_zsum<L, TX>), dim3((unsigned)(ntx * ny)), dim3(TX * threads_per_transform<L>()),
-                     0, st, a);
+  RLZArgs b = a;
+  b.ny = ny;
+  hipLaunchKernelGGL((sk_zsum<L, TX>), dim3((unsigned)column_tiles(a.nbins, TX, ny)),
+                     dim3(TX * threads_per_transform<L>()), 0, st, b);
 }
 
 template <int L>
 void rl_launch_xratio(const RLXArgs& a, int nviews, hipStream_t st) {
-  constexpr int P = fit_pairs<L, 4>();
+  constexpr int P = batch_pairs<L>();
   const long groups = ((a.nrows + 1) / 2 + P - 1) / P;
   hipLaunchKernelGGL((sk_xratio<L, P>), dim3((unsigned)(groups * nviews)),
                      dim3(P * threads_per_transform<L>()), 0, st, a);
@@ -220,7 +222,7 @@ void rl_launch_xratio(const RLXArgs& a, int nviews, hipStream_t st) {
 
 template <int L>
 void rl_launch_xupdate(const RLXArgs& a, hipStream_t st) {
-  constexpr int P = fit_pairs<L, 4>();
+  constexpr int P = batch_pairs<L>();
   const long groups = ((a.nrows + 1) / 2 + P - 1) / P;
   hipLaunchKernelGGL((sk_xupdate<L, P>), dim3((unsigned)groups), dim3(P * threads_per_transform<L>()),
                      0, st, a);

@@ -36,12 +36,29 @@ constexpr int fit_tile() {
   return tx;
 }
 
+// row pairs per x-pass workgroup for about TGT threads (T threads per row
+// pair), within the LDS and thread limits of fit_pairs
+template <int L, int TGT>
+constexpr int pairs_for() {
+  const int t = threads_per_transform<L>();
+  return t >= TGT ? 1 : TGT / t;
+}
+
 template <int L, int WANT>
 constexpr int fit_pairs() {
   int p = WANT;
   while (p > 1 && (p * threads_per_transform<L>() > 1024 || (long)p * (L + L / 8 + 1) * 8 > 65536))
     p /= 2;
   return p;
+}
+
+#ifndef FC_BATCH_XTGT
+#define FC_BATCH_XTGT 256
+#endif
+// row pairs per workgroup of the batched x passes
+template <int L>
+constexpr int batch_pairs() {
+  return fit_pairs<L, pairs_for<L, FC_BATCH_XTGT>()>();
 }
 
 // padded LDS row of the x passes: one float2 of padding per 8 keeps the
@@ -154,6 +171,54 @@ __device__ __forceinline__ void xinv_body(const XPassArgs& a, long blk, float2* 
   });
 }
 
+// Same transform, but every half-spectrum bin of the two rows is read from
+// HBM once (coalesced) into LDS and the Hermitian mirror is formed there.
+template <int L, int PAIRS>
+__device__ __forceinline__ void xinv_lds_body(const XPassArgs& a, long blk, float2* lds) {
+  using P = PlanOf<L, false>;
+  constexpr int T = P::T, LP = XRow<L>::LP;
+  const int t = threadIdx.x % T, c = threadIdx.x / T;
+  const long ra = 2 * (blk * PAIRS + c), rb = ra + 1;
+  const bool oka = ra < a.nrows, okb = rb < a.nrows;
+  const float2* pa = a.csrc + ra * a.src_row;
+  const float2* pb = a.csrc + rb * a.src_row;
+  const int nb = a.nbins;
+  float2* row = lds + c * LP;
+  auto addr = [&](int pos) { return XRow<L>::at(pos); };
+  // Z = A' + iB' over all L positions, built from the nb bins of each row:
+  // position kb takes A + iB, its mirror L - kb takes conj(A) + i conj(B)
+  for (int kb = t; kb < nb; kb += T) {
+    float2 A = make_float2(0.f, 0.f), B = make_float2(0.f, 0.f);
+    if (oka) A = pa[kb];
+    if (okb) B = pb[kb];
+    if (kb == 0 || 2 * kb == L) {
+      A.y = 0.f;
+      B.y = 0.f;
+    }
+    // Z[kb] = A + iB and Z[L-kb] = conj(A) + i conj(B)
+    row[addr(kb)] = make_float2(A.x - B.y, A.y + B.x);
+    if (kb != 0 && 2 * kb != L) row[addr(L - kb)] = make_float2(A.x + B.y, B.x - A.y);
+  }
+  __syncthreads();
+  float2 v[P::MAXE()];
+  sfft_for_inputs<L, false>(t, [&](int slot, int pos) { v[slot] = row[addr(pos)]; });
+  __syncthreads();
+  sfft_run<L, false, +1>(v, t, row, addr, a.fft.stw);
+  float* oa = a.rdst + ra * a.dst_row;
+  float* ob = a.rdst + rb * a.dst_row;
+  sfft_for_outputs<L, false>(t, [&](int slot, int pos) {
+    if (oka) oa[pos] = v[slot].x;
+    if (okb) ob[pos] = v[slot].y;
+  });
+}
+
+template <int L, int PAIRS>
+__global__ void __launch_bounds__(PAIRS * threads_per_transform<L>())
+sk_xinv_lds(XPassArgs a) {
+  __shared__ float2 lds[PAIRS * XRow<L>::LP];
+  xinv_lds_body<L, PAIRS>(a, blockIdx.x, lds);
+}
+
 template <int L, int PAIRS>
 __global__ void __launch_bounds__(PAIRS * threads_per_transform<L>())
 sk_xinv(XPassArgs a) {
@@ -170,8 +235,8 @@ sb_xinv(const XPassArgs* __restrict__ args, const int2* __restrict__ tasks) {
 }
 
 // ---------------------------------------------------------------------------
-// column pass (y): tiles of TX bins x L positions; tile index
-// o * ntx + xt with ntx = ceil(nbins / TX)
+// column pass (y): tiles of TX bins x L positions, numbered by
+// column_tile_decode (kernels.hpp)
 // ---------------------------------------------------------------------------
 template <int L, int TX, bool REV = false>
 struct ColTile {
@@ -182,12 +247,11 @@ struct ColTile {
   bool ok;
 
   __device__ ColTile(const ColArgs& args, long tile, int c) : a(&args) {
-    const int ntx = (args.nbins + TX - 1) / TX;
-    const long o = tile / ntx;
-    const int x = (int)(tile - o * ntx) * TX + c;
-    ok = x < args.nbins;
-    base_src = o * args.src_outer + x;
-    base_dst = o * args.dst_outer + x;
+    long o;
+    int x;
+    column_tile_decode(args.nbins, TX, args.nouter, tile, c, &o, &x, &ok);
+    base_src = ok ? o * args.src_outer + x : 0;
+    base_dst = ok ? o * args.dst_outer + x : 0;
   }
   __device__ void load(int t, float2 (&v)[MAXE]) const {
     const float2* src = a->src + base_src;
@@ -339,10 +403,13 @@ struct ZTile {
   const float2* kcol;
   bool ok;
   __device__ ZTile(const ZArgs& a, long tile, int c) {
-    const int ntx = (a.nbins + TX - 1) / TX;
-    const long y = tile / ntx;
-    const int x = (int)(tile - y * ntx) * TX + c;
-    ok = x < a.nbins;
+    long y;
+    int x;
+    column_tile_decode(a.nbins, TX, a.ny, tile, c, &y, &x, &ok);
+    if (!ok) {
+      y = 0;
+      x = 0;
+    }
     col = a.s + y * a.s_row + x;
     kcol = a.k + y * a.k_row + x;
   }
@@ -442,9 +509,10 @@ sk_zfused_dft(ZArgs a) {
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-template <int L, int PAIRS>
+// x-pass launchers take the target thread count per workgroup (TGT)
+template <int L, int TGT>
 bool sk_launch_xfwd(const XPass2& a, hipStream_t st) {
-  constexpr int P = fit_pairs<L, PAIRS>();
+  constexpr int P = fit_pairs<L, pairs_for<L, TGT>()>();
   XPass2 b = a;
   b.m_blocks = ((a.m.nrows + 1) / 2 + P - 1) / P;
   b.k_blocks = ((a.k.nrows + 1) / 2 + P - 1) / P;
@@ -453,12 +521,21 @@ bool sk_launch_xfwd(const XPass2& a, hipStream_t st) {
   return true;
 }
 
-template <int L, int PAIRS>
+template <int L, int TGT>
 bool sk_launch_xinv(const XPassArgs& a, hipStream_t st) {
-  constexpr int P = fit_pairs<L, PAIRS>();
+  constexpr int P = fit_pairs<L, pairs_for<L, TGT>()>();
   const long blocks = ((a.nrows + 1) / 2 + P - 1) / P;
   hipLaunchKernelGGL((sk_xinv<L, P>), dim3((unsigned)blocks), dim3(P * threads_per_transform<L>()),
                      0, st, a);
+  return true;
+}
+
+template <int L, int TGT>
+bool sk_launch_xinv_lds(const XPassArgs& a, hipStream_t st) {
+  constexpr int P = fit_pairs<L, pairs_for<L, TGT>()>();
+  const long blocks = ((a.nrows + 1) / 2 + P - 1) / P;
+  hipLaunchKernelGGL((sk_xinv_lds<L, P>), dim3((unsigned)blocks),
+                     dim3(P * threads_per_transform<L>()), 0, st, a);
   return true;
 }
 
@@ -466,8 +543,10 @@ template <int L, int TX>
 Col2 col2_tiles(const Col2& a) {
   Col2 b = a;
   b.ntx = (a.m.nbins + TX - 1) / TX;
-  b.m_tiles = (long)b.ntx * a.m_outer;
-  b.ntiles = b.m_tiles + (long)b.ntx * a.k_outer;
+  b.m.nouter = a.m_outer;
+  b.k.nouter = a.k_outer;
+  b.m_tiles = column_tiles(a.m.nbins, TX, a.m_outer);
+  b.ntiles = b.m_tiles + (a.k_outer > 0 ? column_tiles(a.k.nbins, TX, a.k_outer) : 0);
   return b;
 }
 
@@ -503,9 +582,10 @@ bool sk_launch_col_pipe(const Col2& a, int dir, hipStream_t st) {
 template <int L, int WANT>
 bool sk_launch_zfused_fft(const ZArgs& a, long ny, hipStream_t st) {
   constexpr int TX = fit_tile<L, WANT>();
-  const int ntx = (a.nbins + TX - 1) / TX;
-  hipLaunchKernelGGL((sk_zfused_fft<L, TX>), dim3((unsigned)(ntx * ny)),
-                     dim3(TX * threads_per_transform<L>()), 0, st, a);
+  ZArgs b = a;
+  b.ny = ny;
+  hipLaunchKernelGGL((sk_zfused_fft<L, TX>), dim3((unsigned)column_tiles(a.nbins, TX, ny)),
+                     dim3(TX * threads_per_transform<L>()), 0, st, b);
   return true;
 }
 
@@ -520,22 +600,23 @@ template <int L, int WANT>
 bool sk_launch_zfused_dft(const ZArgs& a, long ny, hipStream_t st) {
   if (a.nvalid > dft_kz_limit<L>()) return false;
   constexpr int TX = fit_tile<L, WANT>();
-  const int ntx = (a.nbins + TX - 1) / TX;
-  hipLaunchKernelGGL((sk_zfused_dft<L, TX>), dim3((unsigned)(ntx * ny)),
-                     dim3(TX * threads_per_transform<L>()), 0, st, a);
+  ZArgs b = a;
+  b.ny = ny;
+  hipLaunchKernelGGL((sk_zfused_dft<L, TX>), dim3((unsigned)column_tiles(a.nbins, TX, ny)),
+                     dim3(TX * threads_per_transform<L>()), 0, st, b);
   return true;
 }
 
 // batched launches over device task tables (default variants)
 template <int L>
 void sb_launch_xfwd(const XPassArgs* args, const int2* tasks, int ntasks, hipStream_t st) {
-  constexpr int P = fit_pairs<L, 4>();
+  constexpr int P = batch_pairs<L>();
   hipLaunchKernelGGL((sb_xfwd<L, P>), dim3((unsigned)ntasks), dim3(P * threads_per_transform<L>()),
                      0, st, args, tasks);
 }
 template <int L>
 void sb_launch_xinv(const XPassArgs* args, const int2* tasks, int ntasks, hipStream_t st) {
-  constexpr int P = fit_pairs<L, 4>();
+  constexpr int P = batch_pairs<L>();
   hipLaunchKernelGGL((sb_xinv<L, P>), dim3((unsigned)ntasks), dim3(P * threads_per_transform<L>()),
                      0, st, args, tasks);
 }
@@ -558,14 +639,22 @@ void sb_launch_zfused(const ZArgs* args, const int2* tasks, int ntasks, hipStrea
 template <int L>
 StaticEntry make_static_entry() {
   StaticEntry e;
-  e.xfwd = {{"p4", &sk_launch_xfwd<L, 4>}, {"p2", &sk_launch_xfwd<L, 2>}};
-  e.xinv = {{"p4", &sk_launch_xinv<L, 4>}, {"p2", &sk_launch_xinv<L, 2>}};
+  e.xfwd = {{"w256", &sk_launch_xfwd<L, 256>},
+            {"w128", &sk_launch_xfwd<L, 128>},
+            {"w64", &sk_launch_xfwd<L, 64>}};
+  e.xinv = {{"w256", &sk_launch_xinv<L, 256>},
+            {"w128", &sk_launch_xinv<L, 128>},
+            {"w64", &sk_launch_xinv<L, 64>},
+            {"w256-ldsmirror", &sk_launch_xinv_lds<L, 256>}};
   e.col = {{"t16", &sk_launch_col<L, 16>},
            {"t16-pipe", &sk_launch_col_pipe<L, 16>},
            {"t8-pipe", &sk_launch_col_pipe<L, 8>},
-           {"t16-revradix", &sk_launch_col<L, 16, true>}};
-  e.zfused = {{"t16-fft", &sk_launch_zfused_fft<L, 16>}, {"t16-dft", &sk_launch_zfused_dft<L, 16>}};
-  e.batch.pairs = fit_pairs<L, 4>();
+           {"t16-revradix", &sk_launch_col<L, 16, true>},
+           {"t8", &sk_launch_col<L, 8>}};
+  e.zfused = {{"t16-fft", &sk_launch_zfused_fft<L, 16>},
+              {"t16-dft", &sk_launch_zfused_dft<L, 16>},
+              {"t8-fft", &sk_launch_zfused_fft<L, 8>}};
+  e.batch.pairs = batch_pairs<L>();
   e.batch.tx = fit_tile<L, 16>();
   e.batch.xfwd = &sb_launch_xfwd<L>;
   e.batch.xinv = &sb_launch_xinv<L>;

@@ -65,8 +65,12 @@ FCAMD_API int fcamd_conv_grad_kernel_device(float* d_dkernel, const float* d_gra
  * streams); fcamd_batch_execute enqueues the sequence on `stream`,
  * fcamd_batch_graph_launch captures it into a hipGraph on the first call and
  * replays that graph afterwards.  The buffers are bound at creation.
+ * Blocks run in waves of consecutive blocks whose workspaces add up to at
+ * most FCAMD_BATCH_WAVE_MB MiB (environment, read at creation; unset = the
+ * library default, 0 = a single wave); waves share one workspace region.
  * d_workspace may be NULL (the plan allocates it) or hold
- * fcamd_batch_workspace_bytes() bytes.  create returns NULL on error. */
+ * fcamd_batch_workspace_bytes() bytes (the largest wave's workspace under
+ * the same setting).  create returns NULL on error. */
 FCAMD_API long long fcamd_batch_workspace_bytes(int nblocks, const int* imDims,
                                                 const int* kernelDims);
 FCAMD_API void* fcamd_batch_create(int nblocks, const int* imDims, const int* kernelDims,

@@ -87,3 +87,37 @@ def test_uneven_64_block_workload(fc, oracle):
             checked += 1
     assert checked >= 4
     plan.close()
+
+
+@pytest.mark.parametrize("wave_mb", [1, 24])
+def test_batched_waves(fc, oracle, monkeypatch, wave_mb):
+    """Blocks split into workspace-budget waves sharing one workspace region
+    (1 MiB: one block per wave; 24 MiB: a few blocks per wave): equal to the
+    plain calls, eager and over repeated graph replays."""
+    import ctypes
+    import torch
+    from fourierconvolutioncudalib_amd.batched import BatchedConvolution
+    imgs, ks = _blocks(500, SHAPES)
+    ti = [torch.from_numpy(a).cuda() for a in imgs]
+    tk = [torch.from_numpy(k).cuda() for k in ks]
+    dims = [d for a in imgs for d in a.shape[::-1]]
+    kdims = [d for k in ks for d in k.shape[::-1]]
+    n = len(imgs)
+    lib = fc.lib()
+    one_wave = lib.fcamd_batch_workspace_bytes(n, (ctypes.c_int * (3 * n))(*dims),
+                                               (ctypes.c_int * (3 * n))(*kdims))
+    monkeypatch.setenv("FCAMD_BATCH_WAVE_MB", str(wave_mb))
+    waved = lib.fcamd_batch_workspace_bytes(n, (ctypes.c_int * (3 * n))(*dims),
+                                            (ctypes.c_int * (3 * n))(*kdims))
+    assert 0 < waved < one_wave
+    plan = BatchedConvolution(ti, tk)
+    plain = [fc.convolve3d(a, k).cpu().numpy() for a, k in zip(ti, tk)]
+    eager = [o.cpu().numpy() for o in plan.run(graph=False)]
+    for e, p in zip(eager, plain):
+        assert oracle.rel_l2(e, p) < 1e-6
+    for rep in range(3):
+        got = [o.clone() for o in plan.run(graph=True)]
+        torch.cuda.synchronize()
+        for g, p in zip(got, plain):
+            assert oracle.rel_l2(g.cpu().numpy(), p) < 1e-6, rep
+    plan.close()
