@@ -18,17 +18,27 @@
 
 namespace fcamd {
 
-// loads the compact kernel column of view v (planes placed at (z - cz) mod L)
+// loads the compact kernel column of view v (planes placed at (z - cz) mod
+// L); positions are t + (compile-time constant) after unrolling, so each
+// access is a per-thread base plus a uniform offset
 template <int L, int MAXE>
 __device__ __forceinline__ void load_kernel_column(float2 (&w)[MAXE], int t, bool ok,
                                                    const float2* kcol, long kplane, int kz, int cz) {
+  const float2* kt = kcol + (long)(t + cz) * kplane;
+  const float2* kw = kt - (long)L * kplane;
   sfft_for_inputs<L, false>(t, [&](int slot, int pos) {
-    int j = pos + cz;
-    if (j >= L) j -= L;
+    const bool wrap = pos + cz >= L;
+    const int j = wrap ? pos + cz - L : pos + cz;
     float2 val = make_float2(0.f, 0.f);
-    if (ok && j < kz) val = kcol[(long)j * kplane];
+    if (ok && j < kz) val = (wrap ? kw : kt)[(long)(pos - t) * kplane];
     w[slot] = val;
   });
+}
+
+// column access at position pos through a base at position t
+template <class T>
+__device__ __forceinline__ T& col_at(T* base_t, int pos, int t, long stride) {
+  return base_t[(long)(pos - t) * stride];
 }
 
 template <int L, int TX>
@@ -44,9 +54,10 @@ sk_zfan(RLZArgs a) {
   const long base = ok ? y * a.s_row + x : 0;
   auto addr = [&](int pos) { return pos * TX + c; };
   float2 X[P::MAXE()];
+  const float2* src_t = a.src + base + (long)t * a.s_plane;
   sfft_for_inputs<L, false>(t, [&](int slot, int pos) {
     float2 val = make_float2(0.f, 0.f);
-    if (ok) val = a.src[base + (long)pos * a.s_plane];
+    if (ok) val = col_at(src_t, pos, t, a.s_plane);
     X[slot] = val;
   });
   sfft_run<L, false, -1>(X, t, lds, addr, a.fft.stw);
@@ -56,8 +67,8 @@ sk_zfan(RLZArgs a) {
     sfft_run<L, false, -1>(W, t, lds, addr, a.fft.stw);
     sfft_for_outputs<L, false>(t, [&](int slot, int) { W[slot] = cscale(cmul(X[slot], W[slot]), a.scale); });
     sfft_run<L, true, +1>(W, t, lds, addr, a.fft.stw);
-    float2* out = a.view[v] + base;
-    if (ok) sfft_for_outputs<L, true>(t, [&](int slot, int pos) { out[(long)pos * a.s_plane] = W[slot]; });
+    float2* out_t = a.view[v] + base + (long)t * a.s_plane;
+    if (ok) sfft_for_outputs<L, true>(t, [&](int slot, int pos) { col_at(out_t, pos, t, a.s_plane) = W[slot]; });
   }
 }
 
@@ -78,10 +89,10 @@ sk_zsum(RLZArgs a) {
   for (int i = 0; i < P::MAXE(); ++i) acc[i] = make_float2(0.f, 0.f);
   for (int v = 0; v < a.nviews; ++v) {
     float2 X[P::MAXE()], W[P::MAXE()];
-    const float2* in = a.view[v] + base;
+    const float2* in_t = a.view[v] + base + (long)t * a.s_plane;
     sfft_for_inputs<L, false>(t, [&](int slot, int pos) {
       float2 val = make_float2(0.f, 0.f);
-      if (ok) val = in[(long)pos * a.s_plane];
+      if (ok) val = col_at(in_t, pos, t, a.s_plane);
       X[slot] = val;
     });
     load_kernel_column<L>(W, t, ok, a.kplanes[v] + base, a.s_plane, a.kz[v], a.cz[v]);
@@ -91,8 +102,8 @@ sk_zsum(RLZArgs a) {
   }
   sfft_for_outputs<L, false>(t, [&](int slot, int) { acc[slot] = cscale(acc[slot], a.scale); });
   sfft_run<L, true, +1>(acc, t, lds, addr, a.fft.stw);
-  float2* out = a.dst + base;
-  if (ok) sfft_for_outputs<L, true>(t, [&](int slot, int pos) { out[(long)pos * a.s_plane] = acc[slot]; });
+  float2* out_t = a.dst + base + (long)t * a.s_plane;
+  if (ok) sfft_for_outputs<L, true>(t, [&](int slot, int pos) { col_at(out_t, pos, t, a.s_plane) = acc[slot]; });
 }
 
 // x-C2R (reversed plan: its outputs sit exactly at the forward plan's

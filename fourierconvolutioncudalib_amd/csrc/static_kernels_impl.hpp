@@ -253,24 +253,38 @@ struct ColTile {
     base_src = ok ? o * args.src_outer + x : 0;
     base_dst = ok ? o * args.dst_outer + x : 0;
   }
+  // Positions are t + (compile-time constant) after unrolling, so the
+  // accesses go through a per-thread base at position t and a uniform
+  // offset (pos - t) * stride instead of a 64-bit product per element.
   __device__ void load(int t, float2 (&v)[MAXE]) const {
     const float2* src = a->src + base_src;
     const int center = a->center, nvalid = a->nvalid;
     const long es = a->src_elem;
     const bool okk = ok;
+    if (center == 0 && nvalid == L) {
+      const float2* st = src + (long)t * es;
+      sfft_for_inputs<L, REV>(t, [&](int slot, int pos) {
+        float2 val = make_float2(0.f, 0.f);
+        if (okk) val = st[(long)(pos - t) * es];
+        v[slot] = val;
+      });
+      return;
+    }
+    // compact support placed circularly (kernel planes): j < nvalid < 2^31 / es
+    const int esi = (int)es;
     sfft_for_inputs<L, REV>(t, [&](int slot, int pos) {
       int j = pos + center;
       if (j >= L) j -= L;
       float2 val = make_float2(0.f, 0.f);
-      if (okk && j < nvalid) val = src[(long)j * es];
+      if (okk && j < nvalid) val = src[j * esi];
       v[slot] = val;
     });
   }
   __device__ void store(int t, const float2 (&v)[MAXE]) const {
     if (!ok) return;
-    float2* dst = a->dst + base_dst;
     const long es = a->dst_elem;
-    sfft_for_outputs<L, REV>(t, [&](int slot, int pos) { dst[(long)pos * es] = v[slot]; });
+    float2* dt = a->dst + base_dst + (long)t * es;
+    sfft_for_outputs<L, REV>(t, [&](int slot, int pos) { dt[(long)(pos - t) * es] = v[slot]; });
   }
 };
 
@@ -413,31 +427,49 @@ struct ZTile {
     col = a.s + y * a.s_row + x;
     kcol = a.k + y * a.k_row + x;
   }
+  // per-thread base at position t plus a uniform offset (pos - t) * stride
   __device__ void load(const ZArgs& a, int t, float2 (&v)[MAXE]) const {
     const long sp = a.s_plane;
     const bool okk = ok;
-    float2* cc = col;
+    const float2* ct = col + (long)t * sp;
     sfft_for_inputs<L, false>(t, [&](int slot, int pos) {
       float2 val = make_float2(0.f, 0.f);
-      if (okk) val = cc[(long)pos * sp];
+      if (okk) val = ct[(long)(pos - t) * sp];
       v[slot] = val;
     });
   }
   __device__ void store(const ZArgs& a, int t, const float2 (&v)[MAXE]) const {
     if (!ok) return;
     const long sp = a.s_plane;
-    float2* cc = col;
-    sfft_for_outputs<L, true>(t, [&](int slot, int pos) { cc[(long)pos * sp] = v[slot]; });
+    float2* ct = col + (long)t * sp;
+    sfft_for_outputs<L, true>(t, [&](int slot, int pos) { ct[(long)(pos - t) * sp] = v[slot]; });
+  }
+  // kernel column: taps j < nvalid placed circularly at (j - center) mod L
+  __device__ void load_kernel(const ZArgs& a, int t, float2 (&w)[MAXE]) const {
+    const long kp = a.k_plane;
+    const int center = a.center, nvalid = a.nvalid;
+    const bool okk = ok;
+    // tap j = pos + center (- L when it wraps): base at tap t + center
+    const float2* kt = kcol + (long)(t + center) * kp;
+    const float2* kw = kt - (long)L * kp;
+    sfft_for_inputs<L, false>(t, [&](int slot, int pos) {
+      const bool wrap = pos + center >= L;
+      const int j = wrap ? pos + center - L : pos + center;
+      float2 val = make_float2(0.f, 0.f);
+      if (okk && j < nvalid) val = (wrap ? kw : kt)[(long)(pos - t) * kp];
+      w[slot] = val;
+    });
   }
 };
 
-// product of the forward image spectrum (v) with the kernel spectrum (w)
+// product of the forward image spectrum (v) with the kernel spectrum (w),
+// conjugated for correlation, times the scale: v * (sc*w.x, +-sc*w.y)
 template <int L, int MAXE>
 __device__ __forceinline__ void zproduct(float2 (&v)[MAXE], const float2 (&w)[MAXE], int t,
                                          bool cj, float sc) {
+  const float si = cj ? -sc : sc;
   sfft_for_outputs<L, false>(t, [&](int slot, int) {
-    const float2 p = cj ? cmulc(v[slot], w[slot]) : cmul(v[slot], w[slot]);
-    v[slot] = cscale(p, sc);
+    v[slot] = cmul(v[slot], make_float2(sc * w[slot].x, si * w[slot].y));
   });
 }
 
@@ -451,19 +483,7 @@ __device__ __forceinline__ void zfused_fft_body(const ZArgs& a, long tile_index,
   float2 v[ZT::MAXE];
   tile.load(a, t, v);
   float2 w[ZT::MAXE];
-  {
-    const long kp = a.k_plane;
-    const int center = a.center, nvalid = a.nvalid;
-    const bool ok = tile.ok;
-    const float2* kc = tile.kcol;
-    sfft_for_inputs<L, false>(t, [&](int slot, int pos) {
-      int j = pos + center;
-      if (j >= L) j -= L;
-      float2 val = make_float2(0.f, 0.f);
-      if (ok && j < nvalid) val = kc[(long)j * kp];
-      w[slot] = val;
-    });
-  }
+  tile.load_kernel(a, t, w);
   // every sfft_run ends with a barrier after its last LDS read, so the
   // next transform may reuse the tile
   sfft_run<L, false, -1>(v, t, lds, addr, a.fft.stw);
@@ -650,10 +670,14 @@ StaticEntry make_static_entry() {
            {"t16-pipe", &sk_launch_col_pipe<L, 16>},
            {"t8-pipe", &sk_launch_col_pipe<L, 8>},
            {"t16-revradix", &sk_launch_col<L, 16, true>},
-           {"t8", &sk_launch_col<L, 8>}};
+           {"t8", &sk_launch_col<L, 8>},
+           {"t32", &sk_launch_col<L, 32>},
+           {"t64", &sk_launch_col<L, 64>}};
   e.zfused = {{"t16-fft", &sk_launch_zfused_fft<L, 16>},
               {"t16-dft", &sk_launch_zfused_dft<L, 16>},
-              {"t8-fft", &sk_launch_zfused_fft<L, 8>}};
+              {"t8-fft", &sk_launch_zfused_fft<L, 8>},
+              {"t32-fft", &sk_launch_zfused_fft<L, 32>},
+              {"t64-fft", &sk_launch_zfused_fft<L, 64>}};
   e.batch.pairs = batch_pairs<L>();
   e.batch.tx = fit_tile<L, 16>();
   e.batch.xfwd = &sb_launch_xfwd<L>;
