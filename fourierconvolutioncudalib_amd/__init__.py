@@ -196,6 +196,68 @@ def _stream_handle(stream):
     return ctypes.c_void_p(stream.cuda_stream)
 
 
+#: boundary rules of convolve3d_boundary (FCAMD_BOUNDARY_* in the C header)
+BOUNDARY = {"periodic": 0, "zero": 1, "nearest": 2, "reflect": 3, "mirror": 4}
+
+
+def _boundary_code(boundary):
+    if isinstance(boundary, str):
+        if boundary not in BOUNDARY:
+            raise ValueError(f"unknown boundary {boundary!r}; one of {sorted(BOUNDARY)}")
+        return BOUNDARY[boundary]
+    return int(boundary)
+
+
+def padded_extents(image_shape, kernel_shape, boundary):
+    """C-order transform shape a boundary-mode convolution runs at."""
+    p = (ctypes.c_int * 3)()
+    check(lib().fcamd_padded_extents(ints(tuple(image_shape)[::-1]),
+                                     ints(tuple(kernel_shape)[::-1]),
+                                     _boundary_code(boundary), p))
+    return (p[2], p[1], p[0])
+
+
+def boundary_workspace_bytes(image_shape, kernel_shape, boundary):
+    """Device workspace bytes of convolve3d_boundary for C-order shapes."""
+    r = lib().fcamd_boundary_workspace_bytes(ints(tuple(image_shape)[::-1]),
+                                             ints(tuple(kernel_shape)[::-1]),
+                                             _boundary_code(boundary))
+    if r < 0:
+        check(lib().fcamd_last_error())
+    return int(r)
+
+
+def convolve3d_boundary(image, kernel, boundary="zero", op=CONVOLVE, out=None, workspace=None,
+                        stream=None):
+    """Linear convolution of a torch CUDA float32 image extended beyond its
+    edges by ``boundary`` ('zero', 'nearest', 'reflect', 'mirror'; 'periodic'
+    is convolve3d).  Kernel centred at k/2, any size; ``out`` may be
+    ``image``.  The padding is virtual: no padded copy of the image is
+    made."""
+    import torch
+    if image.dtype != torch.float32 or kernel.dtype != torch.float32:
+        raise TypeError("float32 tensors required")
+    if not (image.is_cuda and kernel.is_cuda):
+        raise TypeError("CUDA tensors required (there is no CPU path)")
+    if image.dim() != 3 or kernel.dim() != 3:
+        raise ValueError("image and kernel must be 3D")
+    image = image.contiguous()
+    kernel = kernel.contiguous()
+    if out is None:
+        out = torch.empty_like(image)
+    elif out.shape != image.shape or not out.is_contiguous():
+        raise ValueError("out must be a contiguous tensor shaped like image")
+    wptr, wbytes = None, 0
+    if workspace is not None:
+        wptr, wbytes = workspace.data_ptr(), workspace.numel() * workspace.element_size()
+    with torch.cuda.device(image.device):
+        check(lib().fcamd_convolve3d_boundary_device(
+            out.data_ptr(), image.data_ptr(), ints(image.shape[::-1]),
+            kernel.data_ptr(), ints(kernel.shape[::-1]), int(op), _boundary_code(boundary),
+            wptr, wbytes, _stream_handle(stream)))
+    return out
+
+
 def convolve3d(image, kernel, op=CONVOLVE, out=None, workspace=None, stream=None):
     """Device-resident convolution of torch CUDA float32 tensors.
 

@@ -68,6 +68,17 @@ ConvShape shape_from(const int* imDim, const int* kernelDim) {
   return s;
 }
 
+// the drop-in kernel centring without the kernel <= image requirement (the
+// boundary-mode paths validate the padded transform shape instead)
+ConvShape raw_shape(const int* imDim, const int* kernelDim) {
+  FC_REQUIRE(imDim != nullptr && kernelDim != nullptr, kInvalidArgument, "null dimension array");
+  ConvShape s{imDim[0], imDim[1], imDim[2], kernelDim[0], kernelDim[1], kernelDim[2],
+              kernelDim[0] / 2, kernelDim[1] / 2, kernelDim[2] / 2};
+  FC_REQUIRE(s.nx >= 1 && s.ny >= 1 && s.nz >= 1 && s.kx >= 1 && s.ky >= 1 && s.kz >= 1,
+             kInvalidArgument, "dimensions must be positive");
+  return s;
+}
+
 Op op_from(int op) {
   FC_REQUIRE(op == FCAMD_CONVOLVE || op == FCAMD_CORRELATE, kInvalidArgument, "unknown op");
   return op == FCAMD_CORRELATE ? Op::kCorrelate : Op::kConvolve;
@@ -93,6 +104,30 @@ void host_convolution(const ConvShape& s, const float* im, const float* kernel, 
   FC_HIP_CHECK(hipMemcpyAsync(dker, kernel, nker, hipMemcpyHostToDevice, pool.stream));
   enqueue_convolution(s, static_cast<float*>(dimg), static_cast<float*>(dimg),
                       static_cast<float*>(dker), op, ws, pool.stream);
+  FC_HIP_CHECK(hipMemcpyAsync(out, dimg, nimg, hipMemcpyDeviceToHost, pool.stream));
+  FC_HIP_CHECK(hipStreamSynchronize(pool.stream));
+}
+
+// host-pointer boundary-mode convolution through the device pool
+void host_convolution_padded(const PaddedProblem& p, const float* im, const float* kernel,
+                             float* out, Op op, int dev) {
+  FC_REQUIRE(im != nullptr && kernel != nullptr && out != nullptr, kInvalidArgument,
+             "null data pointer");
+  DeviceGuard guard(dev);
+  DevicePool& pool = device_pool(dev);
+  std::lock_guard<std::mutex> lk(pool.mu);
+  if (!pool.stream) FC_HIP_CHECK(hipStreamCreateWithFlags(&pool.stream, hipStreamNonBlocking));
+  const ConvShape& s = p.padded;
+  const size_t nimg = sizeof(float) * (size_t)p.vp.n[0] * p.vp.n[1] * p.vp.n[2];
+  const size_t nker = sizeof(float) * (size_t)s.kx * s.ky * s.kz;
+  const size_t nws = workspace_layout(s).total();
+  void* dimg = pool_reserve(reinterpret_cast<void**>(&pool.img), &pool.img_bytes, nimg);
+  void* dker = pool_reserve(reinterpret_cast<void**>(&pool.ker), &pool.ker_bytes, nker);
+  void* ws = pool_reserve(&pool.ws, &pool.ws_bytes, nws);
+  FC_HIP_CHECK(hipMemcpyAsync(dimg, im, nimg, hipMemcpyHostToDevice, pool.stream));
+  FC_HIP_CHECK(hipMemcpyAsync(dker, kernel, nker, hipMemcpyHostToDevice, pool.stream));
+  enqueue_convolution_padded(p, static_cast<float*>(dimg), static_cast<float*>(dimg),
+                             static_cast<float*>(dker), op, ws, pool.stream);
   FC_HIP_CHECK(hipMemcpyAsync(out, dimg, nimg, hipMemcpyDeviceToHost, pool.stream));
   FC_HIP_CHECK(hipStreamSynchronize(pool.stream));
 }
@@ -277,6 +312,59 @@ int fcamd_convolve3d_device(float* d_out, const float* d_image, const int* imDim
       // the pooled workspace is reused by the next call: finish before unlocking
       FC_HIP_CHECK(hipStreamSynchronize(st));
     }
+  });
+}
+
+long long fcamd_boundary_workspace_bytes(const int* imDim, const int* kernelDim, int boundary) {
+  long long r = -1;
+  (void)guarded([&] {
+    r = (long long)workspace_layout(padded_problem(raw_shape(imDim, kernelDim), boundary).padded)
+            .total();
+  });
+  return r;
+}
+
+int fcamd_padded_extents(const int* imDim, const int* kernelDim, int boundary, int* padded) {
+  return guarded([&] {
+    FC_REQUIRE(padded != nullptr, kInvalidArgument, "null output array");
+    const PaddedProblem p = padded_problem(raw_shape(imDim, kernelDim), boundary);
+    padded[0] = p.padded.nx;
+    padded[1] = p.padded.ny;
+    padded[2] = p.padded.nz;
+  });
+}
+
+int fcamd_convolve3d_boundary_device(float* d_out, const float* d_image, const int* imDim,
+                                     const float* d_kernel, const int* kernelDim, int op,
+                                     int boundary, void* d_workspace, long long workspace_bytes,
+                                     void* stream) {
+  return guarded([&] {
+    FC_REQUIRE(d_out && d_image && d_kernel, kInvalidArgument, "null device pointer");
+    const PaddedProblem p = padded_problem(raw_shape(imDim, kernelDim), boundary);
+    const Op o = op_from(op);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const size_t need = workspace_layout(p.padded).total();
+    if (d_workspace) {
+      FC_REQUIRE(workspace_bytes >= (long long)need, kInvalidArgument, "workspace too small");
+      enqueue_convolution_padded(p, d_out, d_image, d_kernel, o, d_workspace, st);
+    } else {
+      int dev = 0;
+      FC_HIP_CHECK(hipGetDevice(&dev));
+      DevicePool& pool = device_pool(dev);
+      std::lock_guard<std::mutex> lk(pool.mu);
+      void* ws = pool_reserve(&pool.ws, &pool.ws_bytes, need);
+      enqueue_convolution_padded(p, d_out, d_image, d_kernel, o, ws, st);
+      // the pooled workspace is reused by the next call: finish before unlocking
+      FC_HIP_CHECK(hipStreamSynchronize(st));
+    }
+  });
+}
+
+int fcamd_convolve3d_boundary_host(float* im, const int* imDim, const float* kernel,
+                                   const int* kernelDim, int op, int boundary, int devCUDA) {
+  return guarded([&] {
+    const PaddedProblem p = padded_problem(raw_shape(imDim, kernelDim), boundary);
+    host_convolution_padded(p, im, kernel, im, op_from(op), devCUDA);
   });
 }
 

@@ -15,6 +15,7 @@
 //   P5  x-C2R of S into the output volume
 // The kernel is never materialised at image size.
 #include "conv_driver.hpp"
+#include "../../include/fourierconvolution_amd.h"
 
 #include <cmath>
 #include <map>
@@ -286,7 +287,7 @@ ColArgs spectrum_column_args(const ConvShape& s, const FftDesc& fy, float2* S) {
 }  // namespace
 
 void pass_x_forward(const ConvShape& s, const float* src, float2* S, const float* kernel,
-                    float2* Kx, hipStream_t st) {
+                    float2* Kx, hipStream_t st, const VPad* vp) {
   ensure_kernel_attributes();
   const int nb = half_bins(s.nx);
   const long nxp = padded_bins(s.nx);
@@ -296,7 +297,7 @@ void pass_x_forward(const ConvShape& s, const float* src, float2* S, const float
   // when a compile-time kernel serves nx)
   {
     PassScope ps(kPassXForward, st);
-    if (!static_xfwd(s, fx, src, S, kernel, Kx, st)) {
+    if (!static_xfwd(s, fx, src, S, kernel, Kx, st, vp)) {
       XPassArgs a{};
       a.fft = fx;
       a.nrows = nrows;
@@ -306,10 +307,12 @@ void pass_x_forward(const ConvShape& s, const float* src, float2* S, const float
       a.dst = S;
       a.src_row = s.nx;
       a.dst_row = nxp;
+      if (vp) a.vp = *vp;
       launch_gk_xfwd(a, (nrows + 1) / 2, st);
       if (kernel) {
         PassScope pk(kPassKernelRows, st);
         XPassArgs k = a;
+        k.vp.on = 0;
         k.nrows = (long)s.ky * s.kz;
         k.src = kernel;
         k.dst = Kx;
@@ -355,8 +358,8 @@ void pass_y_forward(const ConvShape& s, float2* S, const float2* Kx, float2* Kxy
 }
 
 void forward_xy(const ConvShape& s, const float* src, float2* S, const float* kernel, float2* Kx,
-                float2* Kxy, hipStream_t st) {
-  pass_x_forward(s, src, S, kernel, Kx, st);
+                float2* Kxy, hipStream_t st, const VPad* vp) {
+  pass_x_forward(s, src, S, kernel, Kx, st, vp);
   pass_y_forward(s, S, kernel ? Kx : nullptr, Kxy, st);
 }
 
@@ -392,16 +395,17 @@ void pass_y_inverse(const ConvShape& s, float2* S, hipStream_t st) {
   }
 }
 
-void inverse_yx(const ConvShape& s, float2* S, float* dst, hipStream_t st) {
+void inverse_yx(const ConvShape& s, float2* S, float* dst, hipStream_t st, const VPad* vp) {
   pass_y_inverse(s, S, st);
-  pass_x_inverse(s, S, dst, st);
+  pass_x_inverse(s, S, dst, st, vp);
 }
 
-void pass_x_inverse(const ConvShape& s, const float2* S, float* dst, hipStream_t st) {
+void pass_x_inverse(const ConvShape& s, const float2* S, float* dst, hipStream_t st,
+                    const VPad* vp) {
   ensure_kernel_attributes();
   const FftDesc fx = fft_desc(s.nx);
   PassScope ps(kPassXInverse, st);
-  if (!static_xinv(s, fx, S, dst, st)) {
+  if (!static_xinv(s, fx, S, dst, st, vp)) {
     const long nrows = (long)s.ny * s.nz;
     XPassArgs a{};
     a.fft = fx;
@@ -412,8 +416,76 @@ void pass_x_inverse(const ConvShape& s, const float2* S, float* dst, hipStream_t
     a.rdst = dst;
     a.src_row = padded_bins(s.nx);
     a.dst_row = s.nx;
+    if (vp) a.vp = *vp;
     launch_gk_xinv(a, (nrows + 1) / 2, st);
   }
+}
+
+int fast_length(int n) {
+  FC_REQUIRE(n >= 1, kInvalidArgument, "transform length must be positive");
+  int smooth = n;  // smallest 7-smooth length >= n
+  for (;; ++smooth) {
+    int r = smooth;
+    for (int f : {2, 3, 5, 7})
+      while (r % f == 0) r /= f;
+    if (r == 1) break;
+  }
+  // a compile-time length unless it is more than 25% longer
+  int best = 0;
+  int lens[64];
+  const int cnt = fcamd_static_lengths(lens, 64);
+  for (int i = 0; i < cnt && i < 64; ++i)
+    if (lens[i] >= n && (best == 0 || lens[i] < best)) best = lens[i];
+  return best && 4L * best <= 5L * smooth ? best : smooth;
+}
+
+PaddedProblem padded_problem(const ConvShape& s, int boundary) {
+  FC_REQUIRE(boundary >= kMapPeriodic && boundary <= kMapMirror, kInvalidArgument,
+             "unknown boundary mode");
+  FC_REQUIRE(s.nx >= 1 && s.ny >= 1 && s.nz >= 1 && s.kx >= 1 && s.ky >= 1 && s.kz >= 1,
+             kInvalidArgument, "dimensions must be positive");
+  PaddedProblem p{};
+  p.padded = s;
+  VPad& v = p.vp;
+  v.on = 1;
+  const int n[3] = {s.nx, s.ny, s.nz}, k[3] = {s.kx, s.ky, s.kz}, c[3] = {s.cx, s.cy, s.cz};
+  int P[3];
+  for (int d = 0; d < 3; ++d) {
+    v.mode[d] = boundary;
+    v.n[d] = n[d];
+    v.m[d] = n[d];
+    v.off[d] = 0;
+    // image at lo = k - 1 - c: output i reads padded i - (k - 1 - c) .. i + c,
+    // all inside [0, n + k - 1) for i in [lo, lo + n), so nothing wraps
+    if (k[d] > 1 && boundary != kMapPeriodic) {
+      FC_REQUIRE((long)n[d] + k[d] - 1 <= (1L << 30), kUnsupportedSize, "padded extent too large");
+      P[d] = fast_length(n[d] + k[d] - 1);
+      v.lo[d] = k[d] - 1 - c[d];
+    } else {
+      P[d] = n[d];
+      v.lo[d] = 0;
+    }
+  }
+  p.padded.nx = P[0];
+  p.padded.ny = P[1];
+  p.padded.nz = P[2];
+  v.py = P[1];
+  validate_shape(p.padded);
+  return p;
+}
+
+void enqueue_convolution_padded(const PaddedProblem& p, float* d_out, const float* d_image,
+                                const float* d_kernel, Op op, void* ws, hipStream_t st) {
+  const ConvShape& s = p.padded;
+  const WorkspaceLayout wl = workspace_layout(s);
+  char* base = static_cast<char*>(ws);
+  float2* S = reinterpret_cast<float2*>(base);
+  float2* Kx = reinterpret_cast<float2*>(base + wl.spectrum);
+  float2* Kxy = reinterpret_cast<float2*>(base + wl.spectrum + wl.krows);
+  forward_xy(s, d_image, S, d_kernel, Kx, Kxy, st, &p.vp);
+  z_pass(s, S, Kxy, false, op == Op::kCorrelate, st);
+  inverse_yx(s, S, d_out, st, &p.vp);
+  FC_HIP_CHECK(hipGetLastError());
 }
 
 void enqueue_convolution(const ConvShape& s, float* d_out, const float* d_image,

@@ -76,17 +76,34 @@ void enqueue_convolution(const ConvShape& s, float* d_out, const float* d_image,
 //               compact kernel planes (dense = false) or a second spectrum
 //   inverse_yx  S -> real volume (inverse y-FFT, x-C2R), unnormalised
 void forward_xy(const ConvShape& s, const float* src, float2* S, const float* kernel, float2* Kx,
-                float2* Kxy, hipStream_t stream);
-// the single passes forward_xy / inverse_yx are made of
+                float2* Kxy, hipStream_t stream, const VPad* vp = nullptr);
+// the single passes forward_xy / inverse_yx are made of; with `vp` the x
+// passes read a virtually padded image and write its cropped interior
 void pass_x_forward(const ConvShape& s, const float* src, float2* S, const float* kernel,
-                    float2* Kx, hipStream_t stream);
+                    float2* Kx, hipStream_t stream, const VPad* vp = nullptr);
 void pass_y_forward(const ConvShape& s, float2* S, const float2* Kx, float2* Kxy,
                     hipStream_t stream);
 void pass_y_inverse(const ConvShape& s, float2* S, hipStream_t stream);
-void pass_x_inverse(const ConvShape& s, const float2* S, float* dst, hipStream_t stream);
+void pass_x_inverse(const ConvShape& s, const float2* S, float* dst, hipStream_t stream,
+                    const VPad* vp = nullptr);
 void z_pass(const ConvShape& s, float2* S, const float2* operand, bool dense, bool conj,
             hipStream_t stream);
-void inverse_yx(const ConvShape& s, float2* S, float* dst, hipStream_t stream);
+void inverse_yx(const ConvShape& s, float2* S, float* dst, hipStream_t stream,
+                const VPad* vp = nullptr);
+
+// Boundary-mode convolution (FCAMD_BOUNDARY_*): the circular convolution
+// of a virtually padded volume, cropped.  Each axis d with k_d > 1 is padded
+// to a fast length P_d >= n_d + k_d - 1 with the image placed at
+// lo_d = k_d - 1 - c_d, so no output of the interior wraps around.
+struct PaddedProblem {
+  ConvShape padded;  // transform shape (P_x, P_y, P_z) with the kernel
+  VPad vp;
+};
+PaddedProblem padded_problem(const ConvShape& s, int boundary);
+// smallest fast transform length >= n (a compile-time length when one is)
+int fast_length(int n);
+void enqueue_convolution_padded(const PaddedProblem& p, float* d_out, const float* d_image,
+                                const float* d_kernel, Op op, void* ws, hipStream_t stream);
 
 // Gradient of sum(g * conv(im, k)) with respect to the kernel taps:
 // dk[j] = sum_i g[i] im[(i - (j - c)) mod n] on the kernel support.

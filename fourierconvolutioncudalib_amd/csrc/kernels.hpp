@@ -7,6 +7,75 @@ namespace fcamd {
 
 // x pass (rows along the contiguous axis).  Forward reads real rows (`src`)
 // and writes half-spectrum rows (`dst`); inverse reads `csrc`, writes `rdst`.
+// Virtual padding of the x passes.  The transform runs on a padded volume
+// (rows of length L = padded nx, py rows per padded plane); padded element p
+// of axis d reads source element vpad_map(p - lo[d] + off[d], n[d],
+// mode[d]) (zero where the map is undefined), and the inverse writes only
+// output elements p - lo[d] in [0, m[d]) of a dense m[0] x m[1] x m[2]
+// output.  Modes follow FCAMD_BOUNDARY_* of the public header.
+constexpr int kMapPeriodic = 0;
+constexpr int kMapZero = 1;
+constexpr int kMapNearest = 2;
+constexpr int kMapReflect = 3;  // half-sample symmetric: d c b a | a b c d | d c b a
+constexpr int kMapMirror = 4;   // whole-sample symmetric: d c b | a b c d | c b a
+
+struct VPad {
+  int on;      // 0: dense rows, no virtual padding
+  int mode[3];  // x, y, z
+  int n[3];     // source extents
+  int lo[3];    // padded position of element 0
+  int off[3];   // source index offset (slab tiling)
+  int m[3];     // output extents of the cropped inverse
+  int py;       // padded rows per plane
+};
+
+__host__ __device__ inline int vpad_map(int i, int n, int mode, bool* ok) {
+  *ok = true;
+  switch (mode) {
+    case kMapZero:
+      *ok = i >= 0 && i < n;
+      return *ok ? i : 0;
+    case kMapNearest:
+      return i < 0 ? 0 : (i >= n ? n - 1 : i);
+    case kMapReflect: {
+      const int per = 2 * n;
+      int r = i % per;
+      if (r < 0) r += per;
+      return r < n ? r : per - 1 - r;
+    }
+    case kMapMirror: {
+      if (n == 1) return 0;
+      const int per = 2 * n - 2;
+      int r = i % per;
+      if (r < 0) r += per;
+      return r < n ? r : per - r;
+    }
+    default: {
+      int r = i % n;
+      if (r < 0) r += n;
+      return r;
+    }
+  }
+}
+
+// source row of padded row r (row index over py x padded nz); false when the
+// row is outside a zero boundary
+__host__ __device__ inline bool vpad_row(const VPad& v, long r, long* src_row) {
+  const int yp = (int)(r % v.py), zp = (int)(r / v.py);
+  bool oky, okz;
+  const int ys = vpad_map(yp - v.lo[1] + v.off[1], v.n[1], v.mode[1], &oky);
+  const int zs = vpad_map(zp - v.lo[2] + v.off[2], v.n[2], v.mode[2], &okz);
+  *src_row = (long)zs * v.n[1] + ys;
+  return oky && okz;
+}
+
+// output row of padded row r under the crop; false when cropped away
+__host__ __device__ inline bool vpad_out_row(const VPad& v, long r, long* out_row) {
+  const int yo = (int)(r % v.py) - v.lo[1], zo = (int)(r / v.py) - v.lo[2];
+  *out_row = (long)zo * v.m[1] + yo;
+  return yo >= 0 && yo < v.m[1] && zo >= 0 && zo < v.m[2];
+}
+
 struct XPassArgs {
   FftDesc fft;          // length nx
   long nrows;           // rows to transform
@@ -21,6 +90,7 @@ struct XPassArgs {
   int kshift;           // forward only: rows are compact kernel rows
   int klen;             // compact row length (kx)
   int kcenter;          // kx / 2
+  VPad vp;              // image rows: virtual padding / cropped inverse
 };
 
 // column pass along y or z over a dense half spectrum (or from the compact

@@ -42,7 +42,14 @@ gk_xfwd(XPassArgs a) {
       if (x >= L) x -= L;
       ok = x < a.klen;
     }
-    if (ok) {
+    if (ok && a.vp.on && !a.kshift) {
+      // padded image rows: read through the boundary maps
+      bool okx;
+      const int xs = vpad_map(pos - a.vp.lo[0] + a.vp.off[0], a.vp.n[0], a.vp.mode[0], &okx);
+      long sr;
+      if (okx && ra < a.nrows && vpad_row(a.vp, ra, &sr)) va = a.src[sr * a.vp.n[0] + xs];
+      if (okx && rb < a.nrows && vpad_row(a.vp, rb, &sr)) vb = a.src[sr * a.vp.n[0] + xs];
+    } else if (ok) {
       if (ra < a.nrows) va = a.src[ra * a.src_row + x];
       if (rb < a.nrows) vb = a.src[rb * a.src_row + x];
     }
@@ -109,6 +116,15 @@ gk_xinv(XPassArgs a) {
     const int pos = w % L, c = w / L;
     const long ra = 2 * (pair0 + c), rb = ra + 1;
     const float2 v = z[pos + c * LP];
+    if (a.vp.on) {
+      // cropped output: interior rows and columns only
+      const int xo = pos - a.vp.lo[0];
+      if (xo < 0 || xo >= a.vp.m[0]) continue;
+      long orow;
+      if (ra < a.nrows && vpad_out_row(a.vp, ra, &orow)) a.rdst[orow * a.vp.m[0] + xo] = v.x;
+      if (rb < a.nrows && vpad_out_row(a.vp, rb, &orow)) a.rdst[orow * a.vp.m[0] + xo] = v.y;
+      continue;
+    }
     if (ra < a.nrows) a.rdst[ra * a.dst_row + pos] = v.x;
     if (rb < a.nrows) a.rdst[rb * a.dst_row + pos] = v.y;
   }

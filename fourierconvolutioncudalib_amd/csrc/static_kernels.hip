@@ -68,9 +68,10 @@ void run_selected(const std::vector<V>& vs, int sel, A&&... args) {
 }  // namespace
 
 bool static_xfwd(const ConvShape& s, const FftDesc& fx, const float* src, float2* dst,
-                 const float* kernel, float2* krows, hipStream_t st) {
+                 const float* kernel, float2* krows, hipStream_t st, const VPad* vp) {
   StaticEntry* e = static_lookup(s.nx);
   if (!e) return false;
+  if (vp && vp->on && !e->xfwd_vpad) return false;
   XPass2 a{};
   a.m.fft = fx;
   a.m.nrows = (long)s.ny * s.nz;
@@ -93,14 +94,19 @@ bool static_xfwd(const ConvShape& s, const FftDesc& fx, const float* src, float2
   } else {
     a.k.nrows = 0;
   }
+  if (vp && vp->on) {
+    a.m.vp = *vp;
+    return e->xfwd_vpad(a, st);
+  }
   run_selected(e->xfwd, e->selected[kSxFwd], a, st);
   return true;
 }
 
 bool static_xinv(const ConvShape& s, const FftDesc& fx, const float2* src, float* dst,
-                 hipStream_t st) {
+                 hipStream_t st, const VPad* vp) {
   StaticEntry* e = static_lookup(s.nx);
   if (!e) return false;
+  if (vp && vp->on && !e->xinv_crop) return false;
   XPassArgs a{};
   a.fft = fx;
   a.nrows = (long)s.ny * s.nz;
@@ -109,6 +115,10 @@ bool static_xinv(const ConvShape& s, const FftDesc& fx, const float2* src, float
   a.rdst = dst;
   a.src_row = padded_bins(s.nx);
   a.dst_row = s.nx;
+  if (vp && vp->on) {
+    a.vp = *vp;
+    return e->xinv_crop(a, st);
+  }
   run_selected(e->xinv, e->selected[kSxInv], a, st);
   return true;
 }

@@ -9,11 +9,13 @@
 namespace fcamd {
 
 // x-R2C of the image rows; with `kernel` non-null the same launch also
-// transforms the compact kernel's rows into `krows`.
+// transforms the compact kernel's rows into `krows`.  With `vp` the image
+// rows are those of a virtually padded volume (VPad) and the inverse writes
+// only its cropped interior.
 bool static_xfwd(const ConvShape& s, const FftDesc& fx, const float* src, float2* dst,
-                 const float* kernel, float2* krows, hipStream_t st);
+                 const float* kernel, float2* krows, hipStream_t st, const VPad* vp = nullptr);
 bool static_xinv(const ConvShape& s, const FftDesc& fx, const float2* src, float* dst,
-                 hipStream_t st);
+                 hipStream_t st, const VPad* vp = nullptr);
 // y-FFT of the spectrum in place; with `krows` non-null the same launch also
 // turns the kernel rows into the kernel planes `kplanes`.
 bool static_ycol(const ConvShape& s, const FftDesc& fy, float2* S, int dir, const float2* krows,

@@ -110,6 +110,58 @@ __device__ __forceinline__ void xfwd_body(const XPassArgs& a, long blk, float2* 
   }
 }
 
+// Image rows of a virtually padded volume (VPad in kernels.hpp): row and
+// element indices go through the boundary maps, zeros where undefined.
+template <int L, int PAIRS>
+__device__ __forceinline__ void xfwd_vpad_body(const XPassArgs& a, long blk, float2* lds) {
+  using P = PlanOf<L, false>;
+  constexpr int T = P::T, LP = XRow<L>::LP;
+  const int t = threadIdx.x % T, c = threadIdx.x / T;
+  const long ra = 2 * (blk * PAIRS + c), rb = ra + 1;
+  const VPad& vp = a.vp;
+  long sa = 0, sb = 0;
+  const bool oka = ra < a.nrows && vpad_row(vp, ra, &sa);
+  const bool okb = rb < a.nrows && vpad_row(vp, rb, &sb);
+  const float* pa = a.src + sa * vp.n[0];
+  const float* pb = a.src + sb * vp.n[0];
+  float2 v[P::MAXE()];
+  sfft_for_inputs<L, false>(t, [&](int slot, int pos) {
+    bool okx;
+    const int xs = vpad_map(pos - vp.lo[0] + vp.off[0], vp.n[0], vp.mode[0], &okx);
+    float xa = 0.f, xb = 0.f;
+    if (okx) {
+      if (oka) xa = pa[xs];
+      if (okb) xb = pb[xs];
+    }
+    v[slot] = make_float2(xa, xb);
+  });
+  float2* row = lds + c * LP;
+  auto addr = [&](int pos) { return XRow<L>::at(pos); };
+  sfft_run<L, false, -1>(v, t, row, addr, a.fft.stw);
+  sfft_for_outputs<L, false>(t, [&](int slot, int pos) { row[addr(pos)] = v[slot]; });
+  __syncthreads();
+  float2* da = a.dst + ra * a.dst_row;
+  float2* db = a.dst + rb * a.dst_row;
+  const bool wa = ra < a.nrows, wb = rb < a.nrows;
+  for (int kb = t; kb < a.nbins; kb += T) {
+    const float2 Z = row[addr(kb)];
+    const float2 Zm = row[addr(kb == 0 ? 0 : L - kb)];
+    if (wa) da[kb] = make_float2(0.5f * (Z.x + Zm.x), 0.5f * (Z.y - Zm.y));
+    if (wb) db[kb] = make_float2(0.5f * (Z.y + Zm.y), 0.5f * (Zm.x - Z.x));
+  }
+}
+
+// kernel rows first (xfwd_body), then the padded image rows
+template <int L, int PAIRS>
+__global__ void __launch_bounds__(PAIRS * threads_per_transform<L>())
+sk_xfwd_vpad(XPass2 a2) {
+  __shared__ float2 lds[PAIRS * XRow<L>::LP];
+  if ((long)blockIdx.x >= a2.k_blocks)
+    xfwd_vpad_body<L, PAIRS>(a2.m, (long)blockIdx.x - a2.k_blocks, lds);
+  else
+    xfwd_body<L, PAIRS>(a2.k, blockIdx.x, lds);
+}
+
 template <int L, int PAIRS>
 __global__ void __launch_bounds__(PAIRS * threads_per_transform<L>())
 sk_xfwd(XPass2 a2) {
@@ -209,6 +261,54 @@ __device__ __forceinline__ void xinv_lds_body(const XPassArgs& a, long blk, floa
   sfft_for_outputs<L, false>(t, [&](int slot, int pos) {
     if (oka) oa[pos] = v[slot].x;
     if (okb) ob[pos] = v[slot].y;
+  });
+}
+
+// x-C2R of a virtually padded volume writing only the cropped interior
+template <int L, int PAIRS>
+__global__ void __launch_bounds__(PAIRS * threads_per_transform<L>())
+sk_xinv_crop(XPassArgs a) {
+  __shared__ float2 lds[PAIRS * XRow<L>::LP];
+  using P = PlanOf<L, false>;
+  constexpr int T = P::T, LP = XRow<L>::LP;
+  const int t = threadIdx.x % T, c = threadIdx.x / T;
+  const long ra = 2 * ((long)blockIdx.x * PAIRS + c), rb = ra + 1;
+  const VPad& vp = a.vp;
+  long oa = 0, ob = 0;
+  const bool oka = ra < a.nrows && vpad_out_row(vp, ra, &oa);
+  const bool okb = rb < a.nrows && vpad_out_row(vp, rb, &ob);
+  const bool ina = ra < a.nrows, inb = rb < a.nrows;
+  const float2* pa = a.csrc + ra * a.src_row;
+  const float2* pb = a.csrc + rb * a.src_row;
+  const int nb = a.nbins;
+  float2 v[P::MAXE()];
+  sfft_for_inputs<L, false>(t, [&](int slot, int pos) {
+    const bool mirror = pos >= nb;
+    const int kb = mirror ? L - pos : pos;
+    float2 A = make_float2(0.f, 0.f), B = make_float2(0.f, 0.f);
+    if (ina && oka) A = pa[kb];
+    if (inb && okb) B = pb[kb];
+    if (kb == 0 || 2 * kb == L) {
+      A.y = 0.f;
+      B.y = 0.f;
+    }
+    if (mirror) {
+      A.y = -A.y;
+      B.y = -B.y;
+    }
+    v[slot] = make_float2(A.x - B.y, A.y + B.x);
+  });
+  float2* row = lds + c * LP;
+  auto addr = [&](int pos) { return XRow<L>::at(pos); };
+  sfft_run<L, false, +1>(v, t, row, addr, a.fft.stw);
+  float* da = a.rdst + oa * vp.m[0];
+  float* db = a.rdst + ob * vp.m[0];
+  const int x0 = vp.lo[0], x1 = vp.lo[0] + vp.m[0];
+  sfft_for_outputs<L, false>(t, [&](int slot, int pos) {
+    if (pos >= x0 && pos < x1) {
+      if (oka) da[pos - x0] = v[slot].x;
+      if (okb) db[pos - x0] = v[slot].y;
+    }
   });
 }
 
@@ -551,6 +651,26 @@ bool sk_launch_xinv(const XPassArgs& a, hipStream_t st) {
 }
 
 template <int L, int TGT>
+bool sk_launch_xfwd_vpad(const XPass2& a, hipStream_t st) {
+  constexpr int P = fit_pairs<L, pairs_for<L, TGT>()>();
+  XPass2 b = a;
+  b.m_blocks = ((a.m.nrows + 1) / 2 + P - 1) / P;
+  b.k_blocks = ((a.k.nrows + 1) / 2 + P - 1) / P;
+  hipLaunchKernelGGL((sk_xfwd_vpad<L, P>), dim3((unsigned)(b.m_blocks + b.k_blocks)),
+                     dim3(P * threads_per_transform<L>()), 0, st, b);
+  return true;
+}
+
+template <int L, int TGT>
+bool sk_launch_xinv_crop(const XPassArgs& a, hipStream_t st) {
+  constexpr int P = fit_pairs<L, pairs_for<L, TGT>()>();
+  const long blocks = ((a.nrows + 1) / 2 + P - 1) / P;
+  hipLaunchKernelGGL((sk_xinv_crop<L, P>), dim3((unsigned)blocks),
+                     dim3(P * threads_per_transform<L>()), 0, st, a);
+  return true;
+}
+
+template <int L, int TGT>
 bool sk_launch_xinv_lds(const XPassArgs& a, hipStream_t st) {
   constexpr int P = fit_pairs<L, pairs_for<L, TGT>()>();
   const long blocks = ((a.nrows + 1) / 2 + P - 1) / P;
@@ -678,6 +798,8 @@ StaticEntry make_static_entry() {
               {"t8-fft", &sk_launch_zfused_fft<L, 8>},
               {"t32-fft", &sk_launch_zfused_fft<L, 32>},
               {"t64-fft", &sk_launch_zfused_fft<L, 64>}};
+  e.xfwd_vpad = &sk_launch_xfwd_vpad<L, 64>;
+  e.xinv_crop = &sk_launch_xinv_crop<L, 64>;
   e.batch.pairs = batch_pairs<L>();
   e.batch.tx = fit_tile<L, 16>();
   e.batch.xfwd = &sb_launch_xfwd<L>;

@@ -48,6 +48,8 @@ struct StaticEntry {
   std::vector<Variant<XInvFn>> xinv;
   std::vector<Variant<ColFn>> col;
   std::vector<Variant<ZFn>> zfused;
+  XFwdFn xfwd_vpad = nullptr;  // x forward over a virtually padded image
+  XInvFn xinv_crop = nullptr;  // x inverse writing the cropped interior
   int selected[kSNumPasses] = {0, 0, 0, 0};
   StaticBatch batch;
   StaticRL rl;

@@ -57,6 +57,41 @@ FCAMD_API int fcamd_conv_grad_kernel_device(float* d_dkernel, const float* d_gra
                                             const int* kernelDim, void* d_workspace,
                                             long long workspace_bytes, void* stream);
 
+/* Boundary modes (no counterpart in the reference; its header's TODO at
+ * src/convolution3Dfft.h:39,54 asks for padding "to impose the boundary
+ * conditions that you want: mirror, zero, etc.").  The drop-in entry points
+ * are circular (FCAMD_BOUNDARY_PERIODIC).  The other modes give the linear
+ * convolution of the image extended beyond its edges by
+ *   ZERO     zeros                               0 0 | a b c d | 0 0
+ *   NEAREST  the edge value                      a a | a b c d | d d
+ *   REFLECT  half-sample symmetric (edge twice)  b a | a b c d | d c
+ *   MIRROR   whole-sample symmetric              c b | a b c d | c b
+ * (scipy.ndimage's 'constant', 'nearest', 'reflect' and 'mirror'); the
+ * kernel is centred at k/2 as in convolution3DfftCUDAInPlace and may exceed
+ * the image.  Each axis with k > 1 is transformed at a fast length
+ * P >= n + k - 1 (fcamd_padded_extents); the padding is virtual (read
+ * through the boundary map in the first pass, cropped in the last).
+ * op FCAMD_CORRELATE correlates under the same extension (the adjoint for
+ * PERIODIC and ZERO). */
+#define FCAMD_BOUNDARY_PERIODIC 0
+#define FCAMD_BOUNDARY_ZERO 1
+#define FCAMD_BOUNDARY_NEAREST 2
+#define FCAMD_BOUNDARY_REFLECT 3
+#define FCAMD_BOUNDARY_MIRROR 4
+FCAMD_API long long fcamd_boundary_workspace_bytes(const int* imDim, const int* kernelDim,
+                                                   int boundary);
+FCAMD_API int fcamd_padded_extents(const int* imDim, const int* kernelDim, int boundary,
+                                   int* padded);
+FCAMD_API int fcamd_convolve3d_boundary_device(float* d_out, const float* d_image,
+                                               const int* imDim, const float* d_kernel,
+                                               const int* kernelDim, int op, int boundary,
+                                               void* d_workspace, long long workspace_bytes,
+                                               void* stream);
+/* host-pointer form of the above, in place like convolution3DfftCUDAInPlace */
+FCAMD_API int fcamd_convolve3d_boundary_host(float* im, const int* imDim, const float* kernel,
+                                             const int* kernelDim, int op, int boundary,
+                                             int devCUDA);
+
 /* Batched uneven blocks (no counterpart in the reference): `nblocks`
  * convolutions, block b with dims imDims[3b..3b+2] ({x,y,z}), kernel dims
  * kernelDims[3b..3b+2], device buffers d_images[b], d_kernels[b] and

@@ -100,6 +100,79 @@ int fco_circular_conv_direct(const float* image, const int* s,
 }
 
 /* ------------------------------------------------------------------------ */
+/* Boundary-mode convolution, direct FP64 sum.  Same kernel orientation and   */
+/* centring as fco_circular_conv_direct (taps at offset j - k/2, flipped as  */
+/* in tests/test_algorithms.hpp:40-47); image positions outside [0, n) are   */
+/* mapped by the boundary rule instead of wrapping.  The rules are those of  */
+/* the header's FCAMD_BOUNDARY_* (scipy.ndimage 'wrap', 'constant' with 0,   */
+/* 'nearest', 'reflect', 'mirror').  No counterpart in the reference, whose  */
+/* header leaves boundary conditions as a TODO (src/convolution3Dfft.h:39).  */
+/* ------------------------------------------------------------------------ */
+static int bmap(int i, int n, int mode, int* ok) {
+  *ok = 1;
+  switch (mode) {
+    case 1: /* zero */
+      *ok = i >= 0 && i < n;
+      return *ok ? i : 0;
+    case 2: /* nearest */
+      return i < 0 ? 0 : (i >= n ? n - 1 : i);
+    case 3: { /* reflect, period 2n */
+      int r = i % (2 * n);
+      if (r < 0) r += 2 * n;
+      return r < n ? r : 2 * n - 1 - r;
+    }
+    case 4: { /* mirror, period 2n - 2 */
+      int per, r;
+      if (n == 1) return 0;
+      per = 2 * n - 2;
+      r = i % per;
+      if (r < 0) r += per;
+      return r < n ? r : per - r;
+    }
+    default: { /* periodic */
+      int r = i % n;
+      if (r < 0) r += n;
+      return r;
+    }
+  }
+}
+
+int fco_boundary_conv_direct(const float* image, const int* s, const float* kernel,
+                             const int* k, int mode, int correlate, double* out) {
+  int a;
+  const int c0 = k[0] / 2, c1 = k[1] / 2, c2 = k[2] / 2;
+  if (mode < 0 || mode > 4) return -1;
+  for (a = 0; a < 3; ++a)
+    if (k[a] < 1 || s[a] < 1) return -1;
+  for (int iz = 0; iz < s[0]; ++iz)
+    for (int iy = 0; iy < s[1]; ++iy)
+      for (int ix = 0; ix < s[2]; ++ix) {
+        double acc = 0.0;
+        for (int jz = 0; jz < k[0]; ++jz) {
+          int okz, oky, okx;
+          /* convolution reads i - (j - c), correlation i + (j - c) */
+          const int dz = correlate ? jz - c0 : c0 - jz;
+          const int z = bmap(iz + dz, s[0], mode, &okz);
+          if (!okz) continue;
+          for (int jy = 0; jy < k[1]; ++jy) {
+            const int dy = correlate ? jy - c1 : c1 - jy;
+            const int y = bmap(iy + dy, s[1], mode, &oky);
+            if (!oky) continue;
+            for (int jx = 0; jx < k[2]; ++jx) {
+              const int dx = correlate ? jx - c2 : c2 - jx;
+              const int x = bmap(ix + dx, s[2], mode, &okx);
+              if (!okx) continue;
+              acc += (double)kernel[((long)jz * k[1] + jy) * k[2] + jx] *
+                     (double)image[((long)z * s[1] + y) * s[2] + x];
+            }
+          }
+        }
+        out[((long)iz * s[1] + iy) * s[2] + ix] = acc;
+      }
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
 /* FP64 DFT of any length: recursive decimation in time over prime factors.  */
 /* Stands in for cuFFT (closed library; see src/convolution3Dfft.cu:519-547) */
 /* by the textbook definition X[k] = sum_j x[j] exp(sign*2*pi*i*j*k/n).      */

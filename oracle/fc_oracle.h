@@ -46,6 +46,12 @@ int fco_circular_conv_direct(const float* image, const int* image_shape,
 /* In-place FP64 complex DFT of any length n (mixed radix, prime factors by
  * direct summation); data is interleaved re/im, element i at data[2*i*stride].
  * sign = -1 forward (cuFFT's forward convention), +1 inverse; unnormalised. */
+/* Linear convolution (correlate = 0) or correlation (1) of an image
+ * extended beyond its edges by boundary rule `mode` (0 periodic, 1 zero,
+ * 2 nearest, 3 reflect, 4 mirror); kernel centred at k/2, any size. */
+int fco_boundary_conv_direct(const float* image, const int* image_shape, const float* kernel,
+                             const int* kernel_shape, int mode, int correlate, double* out);
+
 void fco_fft1d(double* data, int n, long stride, int sign);
 
 /* FP64 3D complex DFT in place over a C-order [nz][ny][nx] interleaved

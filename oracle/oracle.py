@@ -42,6 +42,9 @@ def _lib():
         lib.fco_fftshift_pad.restype = ctypes.c_int
         lib.fco_circular_conv_direct.argtypes = [_fp, _i3, _fp, _i3, _dp]
         lib.fco_circular_conv_direct.restype = ctypes.c_int
+        lib.fco_boundary_conv_direct.argtypes = [_fp, _i3, _fp, _i3, ctypes.c_int,
+                                                 ctypes.c_int, _dp]
+        lib.fco_boundary_conv_direct.restype = ctypes.c_int
         lib.fco_fft1d.argtypes = [_dp, ctypes.c_int, ctypes.c_long, ctypes.c_int]
         lib.fco_fft3d.argtypes = [_dp, _i3, ctypes.c_int]
         lib.fco_fft_conv.argtypes = [_fp, _i3, _fp, _i3, _dp, ctypes.c_int]
@@ -130,6 +133,25 @@ def circular_conv_direct(image, kernel):
                                          _ptr(out, _dp))
     if rc:
         raise ValueError("kernel larger than image")
+    return out
+
+
+BOUNDARY = {"periodic": 0, "zero": 1, "nearest": 2, "reflect": 3, "mirror": 4}
+
+
+def boundary_conv_direct(image, kernel, mode, correlate=False):
+    """FP64 direct linear convolution (or correlation) of the image extended
+    by boundary rule `mode` (a BOUNDARY name or number); kernel centred at
+    k/2, any kernel size."""
+    image = _f32(image)
+    kernel = _f32(kernel)
+    m = BOUNDARY[mode] if isinstance(mode, str) else int(mode)
+    out = np.empty(image.shape, dtype=np.float64)
+    rc = _lib().fco_boundary_conv_direct(_ptr(image, _fp), _ints(image.shape),
+                                         _ptr(kernel, _fp), _ints(kernel.shape), m,
+                                         1 if correlate else 0, _ptr(out, _dp))
+    if rc:
+        raise ValueError("bad boundary mode or shape")
     return out
 
 

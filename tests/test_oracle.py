@@ -227,3 +227,48 @@ def test_rl_flux_conservation_and_numpy_restatement(oracle):
             acc += oracle.numpy_fft_conv(v / np.maximum(blur, 1e-6), k, conj_kernel=True)
         p = p * acc / len(ks)
     assert oracle.rel_l2(psi, p) < 1e-12
+
+
+# ---------------------------------------------------------------------------
+# boundary-mode oracle (the restatement behind convolve3d_boundary)
+# ---------------------------------------------------------------------------
+_SCIPY_MODE = {"periodic": "grid-wrap", "zero": "constant", "nearest": "nearest",
+               "reflect": "reflect", "mirror": "mirror"}
+
+
+@pytest.mark.parametrize("shape,kshape", [((5, 6, 7), (3, 3, 3)), ((4, 9, 6), (5, 3, 7)),
+                                          ((3, 4, 5), (7, 9, 3)), ((1, 6, 5), (1, 3, 5))])
+@pytest.mark.parametrize("mode", sorted(_SCIPY_MODE))
+def test_boundary_oracle_matches_scipy_ndimage(oracle, shape, kshape, mode):
+    """Independent pin: scipy.ndimage.convolve / correlate in float64 with the
+    same extension rule, including kernels larger than the image."""
+    ndi = pytest.importorskip("scipy.ndimage")
+    rng = np.random.default_rng(sum(shape) * 7 + sum(kshape))
+    a = rng.random(shape).astype(np.float32)
+    k = rng.random(kshape).astype(np.float32)
+    ref = ndi.convolve(a.astype(np.float64), k.astype(np.float64), mode=_SCIPY_MODE[mode], cval=0.0)
+    got = oracle.boundary_conv_direct(a, k, mode)
+    assert np.abs(got - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max())
+    refc = ndi.correlate(a.astype(np.float64), k.astype(np.float64), mode=_SCIPY_MODE[mode],
+                         cval=0.0)
+    gotc = oracle.boundary_conv_direct(a, k, mode, correlate=True)
+    assert np.abs(gotc - refc).max() <= 1e-12 * max(1.0, np.abs(refc).max())
+
+
+def test_boundary_oracle_closed_forms(oracle):
+    """Periodic equals the circular oracle; a constant image stays constant
+    under nearest/reflect/mirror with a unit-sum kernel; zero boundary of a
+    constant image near an edge sums only the kernel taps that land inside."""
+    rng = np.random.default_rng(5)
+    a = rng.random((6, 7, 8)).astype(np.float32)
+    k = rng.random((3, 5, 3)).astype(np.float32)
+    assert np.allclose(oracle.boundary_conv_direct(a, k, "periodic"),
+                       oracle.circular_conv_direct(a, k), rtol=0, atol=1e-12)
+    c = np.full((5, 6, 7), 2.5, np.float32)
+    ku = (k / k.sum()).astype(np.float32)
+    for mode in ("nearest", "reflect", "mirror"):
+        assert np.allclose(oracle.boundary_conv_direct(c, ku, mode), 2.5, rtol=0, atol=1e-6)
+    ones = np.ones((4, 4, 4), np.float32)
+    kk = np.ones((3, 3, 3), np.float32)
+    z = oracle.boundary_conv_direct(ones, kk, "zero")
+    assert z[0, 0, 0] == 8.0 and z[1, 1, 1] == 27.0 and z[0, 1, 1] == 18.0
