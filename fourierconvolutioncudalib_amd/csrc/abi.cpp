@@ -8,6 +8,7 @@
 #include "../../include/convolution3Dfft.h"
 #include "../../include/fourierconvolution_amd.h"
 #include "conv_driver.hpp"
+#include "staging.hpp"
 #include "static_kernels.hpp"
 
 using namespace fcamd;
@@ -100,12 +101,11 @@ void host_convolution(const ConvShape& s, const float* im, const float* kernel, 
   void* dimg = pool_reserve(reinterpret_cast<void**>(&pool.img), &pool.img_bytes, nimg);
   void* dker = pool_reserve(reinterpret_cast<void**>(&pool.ker), &pool.ker_bytes, nker);
   void* ws = pool_reserve(&pool.ws, &pool.ws_bytes, nws);
-  FC_HIP_CHECK(hipMemcpyAsync(dimg, im, nimg, hipMemcpyHostToDevice, pool.stream));
-  FC_HIP_CHECK(hipMemcpyAsync(dker, kernel, nker, hipMemcpyHostToDevice, pool.stream));
+  staged_upload(dev, dimg, im, nimg, pool.stream);
+  staged_upload(dev, dker, kernel, nker, pool.stream);
   enqueue_convolution(s, static_cast<float*>(dimg), static_cast<float*>(dimg),
                       static_cast<float*>(dker), op, ws, pool.stream);
-  FC_HIP_CHECK(hipMemcpyAsync(out, dimg, nimg, hipMemcpyDeviceToHost, pool.stream));
-  FC_HIP_CHECK(hipStreamSynchronize(pool.stream));
+  staged_download(dev, out, dimg, nimg, pool.stream);
 }
 
 // host-pointer boundary-mode convolution through the device pool
@@ -124,12 +124,11 @@ void host_convolution_padded(const PaddedProblem& p, const float* im, const floa
   void* dimg = pool_reserve(reinterpret_cast<void**>(&pool.img), &pool.img_bytes, nimg);
   void* dker = pool_reserve(reinterpret_cast<void**>(&pool.ker), &pool.ker_bytes, nker);
   void* ws = pool_reserve(&pool.ws, &pool.ws_bytes, nws);
-  FC_HIP_CHECK(hipMemcpyAsync(dimg, im, nimg, hipMemcpyHostToDevice, pool.stream));
-  FC_HIP_CHECK(hipMemcpyAsync(dker, kernel, nker, hipMemcpyHostToDevice, pool.stream));
+  staged_upload(dev, dimg, im, nimg, pool.stream);
+  staged_upload(dev, dker, kernel, nker, pool.stream);
   enqueue_convolution_padded(p, static_cast<float*>(dimg), static_cast<float*>(dimg),
                              static_cast<float*>(dker), op, ws, pool.stream);
-  FC_HIP_CHECK(hipMemcpyAsync(out, dimg, nimg, hipMemcpyDeviceToHost, pool.stream));
-  FC_HIP_CHECK(hipStreamSynchronize(pool.stream));
+  staged_download(dev, out, dimg, nimg, pool.stream);
 }
 
 }  // namespace

@@ -1,0 +1,46 @@
+"""Host-pointer entry points on volumes large enough for the pinned staging
+path (chunked, several buffers cycled, sizes not multiples of a chunk): the
+result equals the device-resident path bit for bit."""
+import ctypes
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+FP = ctypes.POINTER(ctypes.c_float)
+
+
+@pytest.mark.parametrize("shape,kshape", [((160, 256, 256), (15, 15, 15)),
+                                          ((161, 257, 255), (9, 7, 5)),
+                                          ((40, 64, 64), (5, 5, 5))])
+def test_inplace_host_entry_equals_device_path(fc, shape, kshape):
+    import torch
+    from fourierconvolutioncudalib_amd import synth
+    from fourierconvolutioncudalib_amd._lib import ints
+    print("host staging", shape, kshape, flush=True)
+    a = synth.uniform(shape, 41 + shape[0])
+    k = synth.random_psf(kshape, 43)
+    dev = fc.convolve3d(torch.from_numpy(a).cuda(), torch.from_numpy(k).cuda()).cpu().numpy()
+    im = np.ascontiguousarray(a.copy())
+    kk = np.ascontiguousarray(k)
+    fc.lib().convolution3DfftCUDAInPlace(im.ctypes.data_as(FP), ints(shape[::-1]),
+                                         kk.ctypes.data_as(FP), ints(kshape[::-1]), 0)
+    assert fc.lib().fcamd_last_error() == 0
+    assert np.array_equal(im, dev)
+
+
+def test_boundary_host_entry_large(fc):
+    import torch
+    from fourierconvolutioncudalib_amd import synth
+    from fourierconvolutioncudalib_amd._lib import check, ints
+    shape, kshape = (150, 200, 180), (11, 9, 13)
+    a = synth.uniform(shape, 77)
+    k = synth.random_psf(kshape, 78)
+    dev = fc.convolve3d_boundary(torch.from_numpy(a).cuda(), torch.from_numpy(k).cuda(),
+                                 "reflect").cpu().numpy()
+    im = np.ascontiguousarray(a.copy())
+    check(fc.lib().fcamd_convolve3d_boundary_host(
+        im.ctypes.data_as(FP), ints(shape[::-1]), np.ascontiguousarray(k).ctypes.data_as(FP),
+        ints(kshape[::-1]), fc.CONVOLVE, fc.BOUNDARY["reflect"], 0))
+    assert np.array_equal(im, dev)
