@@ -138,15 +138,19 @@ __host__ __device__ inline void column_tile_decode(int nbins, int TX, long noute
                                                    long* o, int* x, bool* ok) {
   const int nfull = nbins / TX, rem = nbins - nfull * TX;
   if (rem == 1 && nfull > 0) {
-    const long full = (long)nfull * nouter;
-    if (tile < full) {
-      *o = tile / nfull;
-      *x = (int)(tile - *o * nfull) * TX + c;
-      *ok = true;
-    } else {
-      *o = (tile - full) * TX + c;
+    // the Nyquist tiles (strided single-bin gathers, slower than a full
+    // tile) take the lowest indices so they run under the bulk of the grid
+    // instead of trailing it
+    const long nnyq = (nouter + TX - 1) / TX;
+    if (tile < nnyq) {
+      *o = tile * TX + c;
       *x = nbins - 1;
       *ok = *o < nouter;
+    } else {
+      const long f = tile - nnyq;
+      *o = f / nfull;
+      *x = (int)(f - *o * nfull) * TX + c;
+      *ok = true;
     }
     return;
   }
