@@ -1,6 +1,7 @@
 // abi.cpp -- the exported C ABI: the reference's eleven symbols
 // (src/convolution3Dfft.h:29-84) and the additive fcamd_* entry points.
 // No exception or exit() crosses this boundary.
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -8,6 +9,7 @@
 #include "../../include/convolution3Dfft.h"
 #include "../../include/fourierconvolution_amd.h"
 #include "conv_driver.hpp"
+#include "slabs.hpp"
 #include "staging.hpp"
 #include "static_kernels.hpp"
 
@@ -87,6 +89,16 @@ Op op_from(int op) {
 
 // Host-pointer convolution on device `dev`: H2D, passes, D2H into `out`.
 // `out` may alias `im`.  Serialised per device (pooled buffers).
+// z-slab streaming of large host-pointer calls; FCAMD_NO_SLABS=1 turns it
+// off (single-shot upload, convolution, download)
+bool slabs_enabled() {
+  static const bool on = [] {
+    const char* v = std::getenv("FCAMD_NO_SLABS");
+    return !(v && v[0] && v[0] != '0');
+  }();
+  return on;
+}
+
 void host_convolution(const ConvShape& s, const float* im, const float* kernel, float* out,
                       Op op, int dev) {
   FC_REQUIRE(im != nullptr && kernel != nullptr && out != nullptr, kInvalidArgument,
@@ -97,6 +109,18 @@ void host_convolution(const ConvShape& s, const float* im, const float* kernel, 
   if (!pool.stream) FC_HIP_CHECK(hipStreamCreateWithFlags(&pool.stream, hipStreamNonBlocking));
   const size_t nimg = sizeof(float) * (size_t)s.voxels();
   const size_t nker = sizeof(float) * (size_t)s.kx * s.ky * s.kz;
+  const SlabPlan sp = slab_plan(s);
+  if (sp.nslabs > 0 && slabs_enabled()) {
+    // large volumes: z-slab streaming, both PCIe directions at once
+    void* din = pool_reserve(reinterpret_cast<void**>(&pool.img), &pool.img_bytes, nimg);
+    void* dout = pool_reserve(reinterpret_cast<void**>(&pool.out), &pool.out_bytes, nimg);
+    void* dker = pool_reserve(reinterpret_cast<void**>(&pool.ker), &pool.ker_bytes, nker);
+    void* ws = pool_reserve(&pool.ws, &pool.ws_bytes, slab_workspace_bytes(s, sp));
+    host_convolution_slabs(s, sp, im, kernel, out, op, dev, static_cast<float*>(din),
+                           static_cast<float*>(dout), static_cast<float*>(dker), ws, pool.stream,
+                           &host_copy_parallel);
+    return;
+  }
   const size_t nws = workspace_layout(s).total();
   void* dimg = pool_reserve(reinterpret_cast<void**>(&pool.img), &pool.img_bytes, nimg);
   void* dker = pool_reserve(reinterpret_cast<void**>(&pool.ker), &pool.ker_bytes, nker);

@@ -469,6 +469,7 @@ PaddedProblem padded_problem(const ConvShape& s, int boundary) {
   p.padded.nx = P[0];
   p.padded.ny = P[1];
   p.padded.nz = P[2];
+  for (int d = 0; d < 3; ++d) v.hi[d] = P[d];
   v.py = P[1];
   validate_shape(p.padded);
   return p;

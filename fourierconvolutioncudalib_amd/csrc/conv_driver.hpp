@@ -128,6 +128,8 @@ struct DevicePool {
   size_t img_bytes = 0;
   float* ker = nullptr;
   size_t ker_bytes = 0;
+  float* out = nullptr;  // second volume of the slab pipeline
+  size_t out_bytes = 0;
   hipStream_t stream = nullptr;
 };
 

@@ -26,6 +26,7 @@ struct VPad {
   int lo[3];    // padded position of element 0
   int off[3];   // source index offset (slab tiling)
   int m[3];     // output extents of the cropped inverse
+  int hi[3];    // padded positions >= hi read zero (a slab's window)
   int py;       // padded rows per plane
 };
 
@@ -58,13 +59,22 @@ __host__ __device__ inline int vpad_map(int i, int n, int mode, bool* ok) {
   }
 }
 
+// source index of padded position p of axis d; *ok false where it reads zero
+__host__ __device__ inline int vpad_src(const VPad& v, int d, int p, bool* ok) {
+  if (p >= v.hi[d]) {
+    *ok = false;
+    return 0;
+  }
+  return vpad_map(p - v.lo[d] + v.off[d], v.n[d], v.mode[d], ok);
+}
+
 // source row of padded row r (row index over py x padded nz); false when the
-// row is outside a zero boundary
+// row reads zero
 __host__ __device__ inline bool vpad_row(const VPad& v, long r, long* src_row) {
   const int yp = (int)(r % v.py), zp = (int)(r / v.py);
   bool oky, okz;
-  const int ys = vpad_map(yp - v.lo[1] + v.off[1], v.n[1], v.mode[1], &oky);
-  const int zs = vpad_map(zp - v.lo[2] + v.off[2], v.n[2], v.mode[2], &okz);
+  const int ys = vpad_src(v, 1, yp, &oky);
+  const int zs = vpad_src(v, 2, zp, &okz);
   *src_row = (long)zs * v.n[1] + ys;
   return oky && okz;
 }

@@ -45,7 +45,7 @@ gk_xfwd(XPassArgs a) {
     if (ok && a.vp.on && !a.kshift) {
       // padded image rows: read through the boundary maps
       bool okx;
-      const int xs = vpad_map(pos - a.vp.lo[0] + a.vp.off[0], a.vp.n[0], a.vp.mode[0], &okx);
+      const int xs = vpad_src(a.vp, 0, pos, &okx);
       long sr;
       if (okx && ra < a.nrows && vpad_row(a.vp, ra, &sr)) va = a.src[sr * a.vp.n[0] + xs];
       if (okx && rb < a.nrows && vpad_row(a.vp, rb, &sr)) vb = a.src[sr * a.vp.n[0] + xs];

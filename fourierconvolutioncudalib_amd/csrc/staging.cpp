@@ -105,6 +105,8 @@ Stager& stager(int dev) {
 
 }  // namespace
 
+void host_copy_parallel(void* dst, const void* src, size_t bytes) { copy_pool().copy(dst, src, bytes); }
+
 void staged_upload(int dev, void* d_dst, const void* h_src, size_t bytes, hipStream_t st) {
   if (bytes < kDirect) {
     FC_HIP_CHECK(hipMemcpyAsync(d_dst, h_src, bytes, hipMemcpyHostToDevice, st));

@@ -21,5 +21,7 @@ void staged_upload(int dev, void* d_dst, const void* h_src, size_t bytes, hipStr
 // Copies `bytes` from `d_src` (after the work already on `st`) to pageable
 // `h_dst`; returns when the data is in `h_dst`.
 void staged_download(int dev, void* h_dst, const void* d_src, size_t bytes, hipStream_t st);
+// host memcpy split over the staging copy threads
+void host_copy_parallel(void* dst, const void* src, size_t bytes);
 
 }  // namespace fcamd

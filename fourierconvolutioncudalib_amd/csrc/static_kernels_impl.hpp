@@ -127,7 +127,7 @@ __device__ __forceinline__ void xfwd_vpad_body(const XPassArgs& a, long blk, flo
   float2 v[P::MAXE()];
   sfft_for_inputs<L, false>(t, [&](int slot, int pos) {
     bool okx;
-    const int xs = vpad_map(pos - vp.lo[0] + vp.off[0], vp.n[0], vp.mode[0], &okx);
+    const int xs = vpad_src(vp, 0, pos, &okx);
     float xa = 0.f, xb = 0.f;
     if (okx) {
       if (oka) xa = pa[xs];
