@@ -293,18 +293,77 @@ __device__ __forceinline__ void sfft_for_outputs(int t, F f) {
   }
 }
 
+// Stage s twiddle rows of the thread's butterflies, loaded ahead of use.
+template <int L, bool REV, int s>
+struct StageTw {
+  using P = PlanOf<L, REV>;
+  static constexpr int R = P::R(s), Ns = P::Ns(s), NB = P::NB(s), RP = even_up(R);
+  static constexpr int Q = Ns > 1 ? NB * (RP / 2) : 1;
+  float4 q[Q];
+};
+
+template <int L, bool REV, int s>
+__device__ __forceinline__ void sfft_load_tw(StageTw<L, REV, s>& tw, int t,
+                                             const float2* __restrict__ stw) {
+  using P = PlanOf<L, REV>;
+  using W = StageTw<L, REV, s>;
+  if constexpr (W::Ns > 1) {
+    constexpr int M = P::M(s);
+#pragma unroll
+    for (int i = 0; i < W::NB; ++i) {
+      const int b = t + i * P::T;
+      if ((M % P::T) != 0 && b >= M) continue;
+      const float4* q = reinterpret_cast<const float4*>(stw + P::TWOFF(s) + (b % W::Ns) * W::RP);
+#pragma unroll
+      for (int r2 = 0; r2 < W::RP / 2; ++r2) tw.q[i * (W::RP / 2) + r2] = q[r2];
+    }
+  }
+}
+
+// sfft_compute with the twiddles already in registers
+template <int L, bool REV, int DIR, int s, int MAXE>
+__device__ __forceinline__ void sfft_compute_tw(float2 (&v)[MAXE], int t,
+                                                const StageTw<L, REV, s>& tw) {
+  using P = PlanOf<L, REV>;
+  using W = StageTw<L, REV, s>;
+  constexpr int R = P::R(s), M = P::M(s), NB = P::NB(s);
+#pragma unroll
+  for (int i = 0; i < NB; ++i) {
+    const int b = t + i * P::T;
+    if ((M % P::T) != 0 && b >= M) continue;
+    float2* x = &v[i * R];
+    if constexpr (W::Ns > 1) {
+      float2 w[W::RP];
+#pragma unroll
+      for (int r2 = 0; r2 < W::RP / 2; ++r2) {
+        const float4 p = tw.q[i * (W::RP / 2) + r2];
+        w[2 * r2] = make_float2(p.x, p.y);
+        w[2 * r2 + 1] = make_float2(p.z, p.w);
+      }
+#pragma unroll
+      for (int r = 1; r < R; ++r) x[r] = DIR < 0 ? cmul(x[r], w[r]) : cmulc(x[r], w[r]);
+    }
+    Dft<R, DIR>::run(x);
+  }
+}
+
 // Stages 1..S-1 (stage 0 inputs must already be in v); the LDS region is
-// reused, so the caller must not have other threads reading it.
+// reused, so the caller must not have other threads reading it.  A stage's
+// twiddles are requested before the barriers of its exchange (the barrier
+// waits for LDS traffic only), so their latency overlaps the exchange; the
+// data registers are free at that point.
 template <int L, bool REV, int DIR, int s, int MAXE, class Addr>
 __device__ __forceinline__ void sfft_rest(float2 (&v)[MAXE], int t, float2* lds, Addr addr,
                                           const float2* __restrict__ tw) {
   using P = PlanOf<L, REV>;
   if constexpr (s < P::S) {
     sfft_store_lds<L, REV, s - 1>(v, t, lds, addr);
+    StageTw<L, REV, s> stw;
+    sfft_load_tw<L, REV, s>(stw, t, tw);
     __syncthreads();
     sfft_load_lds<L, REV, s>(v, t, lds, addr);
     __syncthreads();
-    sfft_compute<L, REV, DIR, s>(v, t, tw);
+    sfft_compute_tw<L, REV, DIR, s>(v, t, stw);
     sfft_rest<L, REV, DIR, s + 1>(v, t, lds, addr, tw);
   }
 }
