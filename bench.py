@@ -168,6 +168,29 @@ def rl_comparison(fc, torch):
     return res
 
 
+def boundary_comparison(fc, torch, img, psf, reps=10):
+    """Linear convolution with a boundary rule at the workload shape: the
+    virtually padded transform size and its time per call."""
+    out = {}
+    ws = torch.empty(fc.boundary_workspace_bytes(img.shape, psf.shape, "reflect"),
+                     dtype=torch.uint8, device="cuda")
+    res = torch.empty_like(img)
+    for mode in ("zero", "reflect"):
+        for _ in range(2):
+            fc.convolve3d_boundary(img, psf, mode, out=res, workspace=ws)
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fc.convolve3d_boundary(img, psf, mode, out=res, workspace=ws)
+        e1.record()
+        torch.cuda.synchronize()
+        out[mode + "_ms"] = e0.elapsed_time(e1) / reps
+    out["padded_zyx"] = list(fc.padded_extents(img.shape, psf.shape, "reflect"))
+    return out
+
+
 def load_traffic(workload):
     path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
     if not os.path.exists(path):
@@ -321,6 +344,7 @@ def main():
         "host_api_ms_pcie_inclusive": min(t_host) * 1e3,
     }
     if not args.no_batched:
+        result["boundary_modes"] = boundary_comparison(fc, torch, img, psf)
         result["batched"] = batched_comparison(fc, torch)
         result["richardson_lucy"] = rl_comparison(fc, torch)
     if not args.no_cpu_baseline:

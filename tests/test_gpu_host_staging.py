@@ -49,3 +49,43 @@ def test_boundary_host_entry_large(fc, oracle):
         im.ctypes.data_as(FP), ints(shape[::-1]), np.ascontiguousarray(k).ctypes.data_as(FP),
         ints(kshape[::-1]), fc.CONVOLVE, fc.BOUNDARY["reflect"], 0))
     assert np.array_equal(im, dev)
+
+
+def test_concurrent_host_calls(fc, oracle):
+    """Host-pointer calls from several threads on one device (pooled buffers,
+    pinned staging and the slab streams are shared per device): every result
+    is that of a call made alone."""
+    import threading
+    from fourierconvolutioncudalib_amd import synth
+    from fourierconvolutioncudalib_amd._lib import ints
+    cases = [((96, 192, 200), (9, 7, 5)), ((40, 48, 32), (5, 5, 5)), ((160, 128, 256), (15, 3, 7)),
+             ((24, 20, 16), (3, 3, 3))]
+    inputs = [(synth.uniform(s, 500 + i), synth.random_psf(k, 600 + i)) for i, (s, k) in enumerate(cases)]
+    refs = []
+    for a, k in inputs:
+        im = np.ascontiguousarray(a.copy())
+        fc.lib().convolution3DfftCUDAInPlace(im.ctypes.data_as(FP), ints(a.shape[::-1]),
+                                             np.ascontiguousarray(k).ctypes.data_as(FP),
+                                             ints(k.shape[::-1]), 0)
+        refs.append(im)
+    outs = [None] * (3 * len(inputs))
+    errs = []
+
+    def work(j):
+        a, k = inputs[j % len(inputs)]
+        im = np.ascontiguousarray(a.copy())
+        fc.lib().convolution3DfftCUDAInPlace(im.ctypes.data_as(FP), ints(a.shape[::-1]),
+                                             np.ascontiguousarray(k).ctypes.data_as(FP),
+                                             ints(k.shape[::-1]), 0)
+        if fc.lib().fcamd_last_error() != 0:
+            errs.append(j)
+        outs[j] = im
+
+    ths = [threading.Thread(target=work, args=(j,)) for j in range(len(outs))]
+    for t in ths:
+        t.start()
+    for t in ths:
+        t.join()
+    assert not errs
+    for j, o in enumerate(outs):
+        assert np.array_equal(o, refs[j % len(inputs)]), j
