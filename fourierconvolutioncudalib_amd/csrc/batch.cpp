@@ -50,6 +50,7 @@ struct Block {
 template <class A>
 struct Group {
   int L = 0;
+  int path = 0;  // z pass: 1 = the two-stage tap path (all kz <= sparse_kz)
   StaticEntry* e = nullptr;
   std::vector<A> args;
   std::vector<int2> tasks;
@@ -340,10 +341,15 @@ void BatchPlan::build_wave(Wave& w, int first, int last) {
     }
     const FftDesc fz = fft_desc(kv.first);
     const int TX = e->batch.tx;
-    Group<ZArgs> g;
-    g.L = kv.first;
-    g.e = e;
+    // blocks whose kz fits the tap path form their own launch
+    Group<ZArgs> gs[2];
+    for (int p = 0; p < 2; ++p) {
+      gs[p].L = kv.first;
+      gs[p].e = e;
+      gs[p].path = p;
+    }
     for (int b : kv.second) {
+      Group<ZArgs>& g = gs[e->batch.zfused_sparse && blocks_[b].s.kz <= e->batch.sparse_kz ? 1 : 0];
       const Block& bl = blocks_[b];
       const long nxp = padded_bins(bl.s.nx);
       ZArgs a{};
@@ -363,8 +369,11 @@ void BatchPlan::build_wave(Wave& w, int first, int last) {
       g.args.push_back(a);
       add_tasks(g.tasks, (int)g.args.size() - 1, column_tiles(a.nbins, TX, bl.s.ny));
     }
-    upload(g);
-    w.z.push_back(g);
+    for (Group<ZArgs>& g : gs) {
+      if (g.args.empty()) continue;
+      upload(g);
+      w.z.push_back(g);
+    }
   }
 }
 
@@ -421,7 +430,8 @@ void BatchPlan::enqueue_wave(const Wave& w, hipStream_t st) {
   }
   fork(st, w.z.size(), [&](size_t i, hipStream_t s) {
     auto& g = w.z[i];
-    g.e->batch.zfused(g.d_args, g.d_tasks, (int)g.tasks.size(), s);
+    (g.path == 1 ? g.e->batch.zfused_sparse : g.e->batch.zfused)(g.d_args, g.d_tasks,
+                                                                 (int)g.tasks.size(), s);
   });
   for (int b : w.generic_z) {
     const Block& bl = blocks_[b];

@@ -347,6 +347,43 @@ __device__ __forceinline__ void sfft_compute_tw(float2 (&v)[MAXE], int t,
   }
 }
 
+// The two halves of sfft_compute_tw: the twiddle multiply, then the
+// butterflies (lets two transforms share one twiddle set that dies before
+// either butterfly runs).
+template <int L, bool REV, int DIR, int s, int MAXE>
+__device__ __forceinline__ void sfft_twiddle(float2 (&v)[MAXE], int t, const StageTw<L, REV, s>& tw) {
+  using P = PlanOf<L, REV>;
+  using W = StageTw<L, REV, s>;
+  constexpr int R = P::R(s), M = P::M(s), NB = P::NB(s);
+  if constexpr (W::Ns > 1) {
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+      const int b = t + i * P::T;
+      if ((M % P::T) != 0 && b >= M) continue;
+      float2* x = &v[i * R];
+#pragma unroll
+      for (int r2 = 0; r2 < W::RP / 2; ++r2) {
+        const float4 p = tw.q[i * (W::RP / 2) + r2];
+        const int r0 = 2 * r2, r1 = 2 * r2 + 1;
+        if (r0 > 0 && r0 < R) x[r0] = DIR < 0 ? cmul(x[r0], make_float2(p.x, p.y)) : cmulc(x[r0], make_float2(p.x, p.y));
+        if (r1 < R) x[r1] = DIR < 0 ? cmul(x[r1], make_float2(p.z, p.w)) : cmulc(x[r1], make_float2(p.z, p.w));
+      }
+    }
+  }
+}
+
+template <int L, bool REV, int DIR, int s, int MAXE>
+__device__ __forceinline__ void sfft_butterflies(float2 (&v)[MAXE], int t) {
+  using P = PlanOf<L, REV>;
+  constexpr int R = P::R(s), M = P::M(s), NB = P::NB(s);
+#pragma unroll
+  for (int i = 0; i < NB; ++i) {
+    const int b = t + i * P::T;
+    if ((M % P::T) != 0 && b >= M) continue;
+    Dft<R, DIR>::run(&v[i * R]);
+  }
+}
+
 // Stages 1..S-1 (stage 0 inputs must already be in v); the LDS region is
 // reused, so the caller must not have other threads reading it.  A stage's
 // twiddles are requested before the barriers of its exchange (the barrier

@@ -575,15 +575,103 @@ __device__ __forceinline__ void zproduct(float2 (&v)[MAXE], const float2 (&w)[MA
   });
 }
 
-// kernel column by the same FFT as the image column
+// taps per column the two-stage fast path can hold (its stage-1 radix)
+template <int L>
+constexpr int ktap_slots() {
+  return PlanOf<L, false>::S == 2 ? PlanOf<L, false>::R(1) : 1;
+}
+
+// Two-stage plan (L = R0*R1) with the kz taps within one stage-0 stride
+// (nvalid <= R1): stage-0 butterfly r of the kernel column (positions
+// r + rho*R1) meets at most one tap, rho = 0 (tap r + center) or
+// rho = R0 - 1 (tap r + center - R1), so its output k is that tap times 1 or
+// exp(+2 pi i k / R0).  Output k = b of butterfly r is the kernel's stage-1
+// input slot (i, r) of the thread owning stage-1 butterfly b = t + i*T.
+// Each thread loads at most ceil(kz / T) taps into `ktaps`; the image
+// transform's stage-0 exchange barrier publishes them, and the stage-1
+// inputs are read back by broadcast.  Compared with transforming the
+// kernel column like the image's, this drops its stage-0 butterflies, its
+// per-slot predicated loads and one LDS exchange (two per tile remain).
 template <int L, int TX>
-__device__ __forceinline__ void zfused_fft_body(const ZArgs& a, long tile_index, float2* lds) {
+__device__ __forceinline__ void zfused_sparse_body(const ZArgs& a, const ZTile<L, TX>& tile, int t,
+                                                   int c, float2 (&v)[ZTile<L, TX>::MAXE],
+                                                   float2* lds, float2* ktaps) {
+  using P = PlanOf<L, false>;
+  constexpr int R1 = P::R(1), M1 = P::M(1), NB1 = P::NB(1), T = P::T;
+  constexpr int MAXE = ZTile<L, TX>::MAXE;
+  auto addr = [&](int pos) { return pos * TX + c; };
+  const int center = a.center, nvalid = a.nvalid;
+  for (int j = t; j < nvalid; j += T)
+    ktaps[j * TX + c] = tile.ok ? tile.kcol[(long)j * a.k_plane] : make_float2(0.f, 0.f);
+  // exp(+2 pi i b / R0) = conj(tw[b * R1]), tw[j] = exp(-2 pi i j / L)
+  float2 e[NB1];
+#pragma unroll
+  for (int i = 0; i < NB1; ++i) {
+    const int b = t + i * T;
+    e[i] = (M1 % T) != 0 && b >= M1 ? make_float2(1.f, 0.f) : cconj(a.fft.tw[b * R1]);
+  }
+  // image: stage 0 and its exchange; the barrier also publishes the taps
+  sfft_compute<L, false, -1, 0>(v, t, a.fft.stw);
+  sfft_store_lds<L, false, 0>(v, t, lds, addr);
+  StageTw<L, false, 1> tw1;
+  sfft_load_tw<L, false, 1>(tw1, t, a.fft.stw);
+  __syncthreads();
+  sfft_load_lds<L, false, 1>(v, t, lds, addr);
+  // every thread is done with the exchange tile before the inverse reuses it
+  // (the taps live in their own array)
+  __syncthreads();
+  sfft_twiddle<L, false, -1, 1>(v, t, tw1);
+  sfft_butterflies<L, false, -1, 1>(v, t);
+  // the kernel column's stage-1 inputs, built only now to keep them out of
+  // the image transform's register peak; same butterflies, same twiddles
+  float2 w[MAXE];
+#pragma unroll
+  for (int i = 0; i < NB1; ++i) {
+#pragma unroll
+    for (int r = 0; r < R1; ++r) {
+      const int j0 = r + center, j1 = r + center - R1;
+      float2 val = make_float2(0.f, 0.f);
+      if (j0 < nvalid)
+        val = ktaps[j0 * TX + c];
+      else if (j1 >= 0)
+        val = cmul(ktaps[j1 * TX + c], e[i]);
+      w[i * R1 + r] = val;
+    }
+  }
+  sfft_twiddle<L, false, -1, 1>(w, t, tw1);
+  sfft_butterflies<L, false, -1, 1>(w, t);
+  zproduct<L>(v, w, t, a.conj != 0, a.scale);
+  sfft_run<L, true, +1>(v, t, lds, addr, a.fft.stw);
+  tile.store(a, t, v);
+}
+
+// whether the two-stage tap path applies to this launch
+template <int L>
+__host__ __device__ constexpr bool zfused_sparse_ok(int nvalid) {
+  return PlanOf<L, false>::S == 2 && nvalid <= PlanOf<L, false>::R(1);
+}
+
+// Fused z pass of one tile.  PATH 0: kernel column transformed like the
+// image's; 1: the two-stage tap path (caller guarantees zfused_sparse_ok);
+// 2: chosen per tile at run time (batched task lists mix kernels).  Paths 0
+// and 1 are separate instantiations: compiled into one kernel their register
+// peaks add up (~144 against ~105 VGPRs each).
+template <int L, int TX, int PATH>
+__device__ __forceinline__ void zfused_fft_body(const ZArgs& a, long tile_index, float2* lds,
+                                                float2* ktaps) {
   using ZT = ZTile<L, TX>;
   const int c = threadIdx.x % TX, t = threadIdx.x / TX;
   const ZT tile(a, tile_index, c);
   auto addr = [&](int pos) { return pos * TX + c; };
   float2 v[ZT::MAXE];
   tile.load(a, t, v);
+  if constexpr (PlanOf<L, false>::S == 2 && PATH != 0) {
+    if (PATH == 1 || a.nvalid <= PlanOf<L, false>::R(1)) {
+      zfused_sparse_body<L, TX>(a, tile, t, c, v, lds, ktaps);
+      return;
+    }
+  }
+  (void)ktaps;
   float2 w[ZT::MAXE];
   tile.load_kernel(a, t, w);
   // every sfft_run ends with a barrier after its last LDS read, so the
@@ -595,19 +683,21 @@ __device__ __forceinline__ void zfused_fft_body(const ZArgs& a, long tile_index,
   tile.store(a, t, v);
 }
 
-template <int L, int TX>
+template <int L, int TX, int PATH>
 __global__ void __launch_bounds__(TX * threads_per_transform<L>())
 sk_zfused_fft(ZArgs a) {
   __shared__ float2 lds[L * TX];
-  zfused_fft_body<L, TX>(a, blockIdx.x, lds);
+  __shared__ float2 ktaps[TX * (PATH == 0 ? 1 : ktap_slots<L>())];
+  zfused_fft_body<L, TX, PATH>(a, blockIdx.x, lds, ktaps);
 }
 
-template <int L, int TX>
+template <int L, int TX, int PATH>
 __global__ void __launch_bounds__(TX * threads_per_transform<L>())
 sb_zfused(const ZArgs* __restrict__ args, const int2* __restrict__ tasks) {
   __shared__ float2 lds[L * TX];
+  __shared__ float2 ktaps[TX * (PATH == 0 ? 1 : ktap_slots<L>())];
   const int2 tk = tasks[blockIdx.x];
-  zfused_fft_body<L, TX>(args[tk.x], tk.y, lds);
+  zfused_fft_body<L, TX, PATH>(args[tk.x], tk.y, lds, ktaps);
 }
 
 template <int L, int TX>
@@ -726,8 +816,14 @@ bool sk_launch_zfused_fft(const ZArgs& a, long ny, hipStream_t st) {
   constexpr int TX = fit_tile<L, WANT>();
   ZArgs b = a;
   b.ny = ny;
-  hipLaunchKernelGGL((sk_zfused_fft<L, TX>), dim3((unsigned)column_tiles(a.nbins, TX, ny)),
-                     dim3(TX * threads_per_transform<L>()), 0, st, b);
+  const dim3 grid((unsigned)column_tiles(a.nbins, TX, ny)), block(TX * threads_per_transform<L>());
+  if constexpr (PlanOf<L, false>::S == 2) {
+    if (zfused_sparse_ok<L>(a.nvalid)) {
+      hipLaunchKernelGGL((sk_zfused_fft<L, TX, 1>), grid, block, 0, st, b);
+      return true;
+    }
+  }
+  hipLaunchKernelGGL((sk_zfused_fft<L, TX, 0>), grid, block, 0, st, b);
   return true;
 }
 
@@ -774,8 +870,15 @@ void sb_launch_col(const ColArgs* args, const int2* tasks, int ntasks, int dir, 
 template <int L>
 void sb_launch_zfused(const ZArgs* args, const int2* tasks, int ntasks, hipStream_t st) {
   constexpr int TX = fit_tile<L, 16>();
-  hipLaunchKernelGGL((sb_zfused<L, TX>), dim3((unsigned)ntasks), dim3(TX * threads_per_transform<L>()),
-                     0, st, args, tasks);
+  hipLaunchKernelGGL((sb_zfused<L, TX, 0>), dim3((unsigned)ntasks),
+                     dim3(TX * threads_per_transform<L>()), 0, st, args, tasks);
+}
+template <int L>
+void sb_launch_zfused_sparse(const ZArgs* args, const int2* tasks, int ntasks, hipStream_t st) {
+  constexpr int TX = fit_tile<L, 16>();
+  if constexpr (PlanOf<L, false>::S == 2)
+    hipLaunchKernelGGL((sb_zfused<L, TX, 1>), dim3((unsigned)ntasks),
+                       dim3(TX * threads_per_transform<L>()), 0, st, args, tasks);
 }
 
 template <int L>
@@ -808,6 +911,10 @@ StaticEntry make_static_entry() {
   e.batch.xinv = &sb_launch_xinv<L>;
   e.batch.col = &sb_launch_col<L>;
   e.batch.zfused = &sb_launch_zfused<L>;
+  if constexpr (PlanOf<L, false>::S == 2) {
+    e.batch.zfused_sparse = &sb_launch_zfused_sparse<L>;
+    e.batch.sparse_kz = PlanOf<L, false>::R(1);
+  }
   return e;
 }
 

@@ -33,6 +33,9 @@ struct StaticBatch {
   void (*xinv)(const XPassArgs*, const int2*, int, hipStream_t) = nullptr;
   void (*col)(const ColArgs*, const int2*, int, int dir, hipStream_t) = nullptr;
   void (*zfused)(const ZArgs*, const int2*, int, hipStream_t) = nullptr;
+  // the two-stage tap path; every task's kz must be <= sparse_kz
+  void (*zfused_sparse)(const ZArgs*, const int2*, int, hipStream_t) = nullptr;
+  int sparse_kz = 0;
 };
 
 // fused Richardson-Lucy kernels (rl_kernels.hpp)
