@@ -103,11 +103,16 @@ SlabPlan slab_plan(const ConvShape& s) {
       p.m = m;
     }
   }
-  if (p.m > 0) p.nslabs = (s.nz + p.m - 1) / p.m;
+  if (p.m > 0) {
+    // balanced slabs, all transformed at one compile-time length
+    p.nslabs = (s.nz + p.m - 1) / p.m;
+    p.m = (s.nz + p.nslabs - 1) / p.nslabs;
+    p.P = fast_length(p.m + s.kz - 1);
+  }
   return p;
 }
 
-PaddedProblem slab_problem(const ConvShape& s, int z0, int m) {
+PaddedProblem slab_problem(const ConvShape& s, int z0, int m, int P) {
   PaddedProblem p{};
   p.padded = s;
   VPad& v = p.vp;
@@ -123,7 +128,8 @@ PaddedProblem slab_problem(const ConvShape& s, int z0, int m) {
   v.lo[2] = s.kz - 1 - s.cz;
   v.off[2] = z0;
   v.m[2] = m;
-  p.padded.nz = fast_length(m + s.kz - 1);
+  FC_REQUIRE(P >= m + s.kz - 1, kInternal, "slab transform too short");
+  p.padded.nz = P;
   // the window the slab needs; planes beyond it read zero (they may not
   // have been uploaded yet)
   v.hi[0] = s.nx;
@@ -138,7 +144,7 @@ size_t slab_workspace_bytes(const ConvShape& s, const SlabPlan& sp) {
   size_t w = 0;
   for (int i = 0; i < sp.nslabs; ++i) {
     const int z0 = i * sp.m, m = std::min(sp.m, s.nz - z0);
-    w = std::max(w, workspace_layout(slab_problem(s, z0, m).padded).total());
+    w = std::max(w, workspace_layout(slab_problem(s, z0, m, sp.P).padded).total());
   }
   return w;
 }
@@ -213,7 +219,7 @@ void host_convolution_slabs(const ConvShape& s, const SlabPlan& sp, const float*
         }
       if (!ready) break;
       FC_HIP_CHECK(hipStreamWaitEvent(compute, ev_up.get(last), 0));
-      const PaddedProblem p = slab_problem(s, z0, m);
+      const PaddedProblem p = slab_problem(s, z0, m, sp.P);
       enqueue_convolution_padded(p, d_out + (size_t)z0 * s.nx * s.ny, d_in, d_ker, op, ws,
                                  compute);
       FC_HIP_CHECK(hipEventRecord(ev_conv.get(slab_next), compute));

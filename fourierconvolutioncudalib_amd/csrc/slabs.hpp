@@ -10,14 +10,14 @@ namespace fcamd {
 constexpr size_t kSlabMinBytes = size_t(32) << 20;
 
 struct SlabPlan {
-  int P = 0;       // padded z length of a full slab
-  int m = 0;       // output planes per slab
+  int P = 0;       // padded z length of every slab
+  int m = 0;       // output planes per slab (the last may have fewer)
   int nslabs = 0;  // 0: slabs do not apply
 };
 
 SlabPlan slab_plan(const ConvShape& s);
-// the padded problem of output planes [z0, z0 + m)
-PaddedProblem slab_problem(const ConvShape& s, int z0, int m);
+// the padded problem of output planes [z0, z0 + m), transformed at z length P
+PaddedProblem slab_problem(const ConvShape& s, int z0, int m, int P);
 size_t slab_workspace_bytes(const ConvShape& s, const SlabPlan& sp);
 
 // Convolves host `im` into host `out` (may alias) slab by slab.  d_in and
