@@ -168,6 +168,25 @@ def rl_comparison(fc, torch):
     return res
 
 
+def plan_timing(fc, torch, img, psf, steps):
+    """The same convolution through a plan (kernel rows/planes transformed
+    once, fcamd_conv_plan_*): milliseconds per call."""
+    plan = fc.ConvolutionPlan(img.shape, psf)
+    out = torch.empty_like(img)
+    for _ in range(3):
+        plan.run(img, out=out)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        plan.run(img, out=out)
+    e1.record()
+    torch.cuda.synchronize()
+    plan.close()
+    return e0.elapsed_time(e1) / steps
+
+
 def boundary_comparison(fc, torch, img, psf, reps=10):
     """Linear convolution with a boundary rule at the workload shape: the
     virtually padded transform size and its time per call."""
@@ -343,6 +362,7 @@ def main():
         "wall_s_timed_region": wall,
         "host_api_ms_pcie_inclusive": min(t_host) * 1e3,
     }
+    result["plan_ms_per_step"] = plan_timing(fc, torch, img, psf, args.steps)
     if not args.no_batched:
         result["boundary_modes"] = boundary_comparison(fc, torch, img, psf)
         result["batched"] = batched_comparison(fc, torch)

@@ -196,6 +196,67 @@ def _stream_handle(stream):
     return ctypes.c_void_p(stream.cuda_stream)
 
 
+class ConvolutionPlan:
+    """One kernel, many images of one shape: the kernel's rows and planes
+    are transformed once (fcamd_conv_plan_create) and every run() executes
+    only the image's passes.  Results equal convolve3d with the same kernel
+    and op.  Runs of one plan share its workspace: keep them on one stream
+    (or synchronise between streams)."""
+
+    def __init__(self, image_shape, kernel, op=CONVOLVE, stream=None, workspace=None):
+        import torch
+        if kernel.dtype != torch.float32 or not kernel.is_cuda or kernel.dim() != 3:
+            raise TypeError("kernel must be a 3D float32 CUDA tensor")
+        self.image_shape = tuple(int(d) for d in image_shape)
+        self.kernel_shape = tuple(kernel.shape)
+        self.device = kernel.device
+        self._kernel = kernel.contiguous()
+        self._workspace = workspace
+        wptr, wbytes = None, 0
+        if workspace is not None:
+            wptr, wbytes = workspace.data_ptr(), workspace.numel() * workspace.element_size()
+        with torch.cuda.device(self.device):
+            self._plan = lib().fcamd_conv_plan_create(
+                ints(self.image_shape[::-1]), ints(self.kernel_shape[::-1]),
+                self._kernel.data_ptr(), int(op), wptr, wbytes, _stream_handle(stream))
+        if not self._plan:
+            check(lib().fcamd_last_error())
+
+    def set_kernel(self, kernel, stream=None):
+        """Replace the kernel (same dims); transformed on ``stream``."""
+        import torch
+        if tuple(kernel.shape) != self.kernel_shape or kernel.dtype != torch.float32:
+            raise ValueError("kernel must keep the plan's float32 kernel shape")
+        self._kernel = kernel.contiguous()
+        with torch.cuda.device(self.device):
+            check(lib().fcamd_conv_plan_set_kernel(self._plan, self._kernel.data_ptr(),
+                                                   _stream_handle(stream)))
+
+    def run(self, image, out=None, stream=None):
+        """Convolve one image of the plan's shape; ``out`` may be ``image``."""
+        import torch
+        if tuple(image.shape) != self.image_shape or image.dtype != torch.float32:
+            raise ValueError("image must have the plan's shape and dtype float32")
+        image = image.contiguous()
+        if out is None:
+            out = torch.empty_like(image)
+        with torch.cuda.device(self.device):
+            check(lib().fcamd_conv_plan_execute(self._plan, out.data_ptr(), image.data_ptr(),
+                                                _stream_handle(stream)))
+        return out
+
+    def close(self):
+        if getattr(self, "_plan", None):
+            lib().fcamd_conv_plan_destroy(self._plan)
+            self._plan = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 #: boundary rules of convolve3d_boundary (FCAMD_BOUNDARY_* in the C header)
 BOUNDARY = {"periodic": 0, "zero": 1, "nearest": 2, "reflect": 3, "mirror": 4}
 

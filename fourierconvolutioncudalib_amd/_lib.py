@@ -45,6 +45,10 @@ SIGNATURES = {
     "fcamd_radix_plan": (_i, [_i, _ip, _i]),
     "fcamd_grad_kernel_workspace_bytes": (_ll, [_ip, _ip]),
     "fcamd_conv_grad_kernel_device": (_i, [_vp, _vp, _vp, _ip, _ip, _vp, _ll, _vp]),
+    "fcamd_conv_plan_create": (_vp, [_ip, _ip, _vp, _i, _vp, _ll, _vp]),
+    "fcamd_conv_plan_set_kernel": (_i, [_vp, _vp, _vp]),
+    "fcamd_conv_plan_execute": (_i, [_vp, _vp, _vp, _vp]),
+    "fcamd_conv_plan_destroy": (None, [_vp]),
     "fcamd_boundary_workspace_bytes": (_ll, [_ip, _ip, _i]),
     "fcamd_padded_extents": (_i, [_ip, _ip, _i, _ip]),
     "fcamd_convolve3d_boundary_device": (_i, [_vp, _vp, _ip, _vp, _ip, _i, _i, _vp, _ll, _vp]),

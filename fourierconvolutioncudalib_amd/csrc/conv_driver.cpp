@@ -308,7 +308,7 @@ void pass_x_forward(const ConvShape& s, const float* src, float2* S, const float
       a.src_row = s.nx;
       a.dst_row = nxp;
       if (vp) a.vp = *vp;
-      launch_gk_xfwd(a, (nrows + 1) / 2, st);
+      if (src) launch_gk_xfwd(a, (nrows + 1) / 2, st);
       if (kernel) {
         PassScope pk(kPassKernelRows, st);
         XPassArgs k = a;
@@ -336,7 +336,7 @@ void pass_y_forward(const ConvShape& s, float2* S, const float2* Kx, float2* Kxy
   {
     PassScope ps(kPassYForward, st);
     if (!static_ycol(s, fy, S, -1, Kx, Kxy, st)) {
-      if (s.ny > 1) launch_gk_col(spectrum_column_args(s, fy, S), -1, s.nz, st);
+      if (S && s.ny > 1) launch_gk_col(spectrum_column_args(s, fy, S), -1, s.nz, st);
       if (kernel) {
         PassScope pk(kPassKernelPlanes, st);
         ColArgs a{};

@@ -74,7 +74,7 @@ bool static_xfwd(const ConvShape& s, const FftDesc& fx, const float* src, float2
   if (vp && vp->on && !e->xfwd_vpad) return false;
   XPass2 a{};
   a.m.fft = fx;
-  a.m.nrows = (long)s.ny * s.nz;
+  a.m.nrows = src ? (long)s.ny * s.nz : 0;  // no image: the kernel's rows only
   a.m.nbins = half_bins(s.nx);
   a.m.src = src;
   a.m.dst = dst;
@@ -137,7 +137,7 @@ bool static_ycol(const ConvShape& s, const FftDesc& fy, float2* S, int dir, cons
   a.m.src_elem = a.m.dst_elem = nxp;
   a.m.nvalid = s.ny;
   a.m.center = 0;
-  a.m_outer = s.nz;
+  a.m_outer = S ? s.nz : 0;  // no spectrum: the kernel's planes only
   a.k = a.m;
   a.k_outer = 0;
   if (krows) {

@@ -92,6 +92,23 @@ FCAMD_API int fcamd_convolve3d_boundary_host(float* im, const int* imDim, const 
                                              const int* kernelDim, int op, int boundary,
                                              int devCUDA);
 
+/* Convolution plans (the kernel-spectrum reuse the reference leaves as a
+ * TODO, src/convolution3Dfft.cu:265,363 "reuse the results").  create
+ * validates the shapes, takes a device workspace (NULL: the plan allocates
+ * fcamd_workspace_bytes()), and transforms the kernel's rows and planes
+ * once on `stream`; execute runs only the image's passes (same result as
+ * fcamd_convolve3d_device with this kernel and op; d_out may alias
+ * d_image); set_kernel replaces the kernel (same kernel dims).  Calls on one
+ * plan share its workspace and must be ordered (one stream, or
+ * synchronised).  create returns NULL on error. */
+FCAMD_API void* fcamd_conv_plan_create(const int* imDim, const int* kernelDim,
+                                       const float* d_kernel, int op, void* d_workspace,
+                                       long long workspace_bytes, void* stream);
+FCAMD_API int fcamd_conv_plan_set_kernel(void* plan, const float* d_kernel, void* stream);
+FCAMD_API int fcamd_conv_plan_execute(void* plan, float* d_out, const float* d_image,
+                                      void* stream);
+FCAMD_API void fcamd_conv_plan_destroy(void* plan);
+
 /* Batched uneven blocks (no counterpart in the reference): `nblocks`
  * convolutions, block b with dims imDims[3b..3b+2] ({x,y,z}), kernel dims
  * kernelDims[3b..3b+2], device buffers d_images[b], d_kernels[b] and
