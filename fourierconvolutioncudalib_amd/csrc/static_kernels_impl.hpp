@@ -4,8 +4,11 @@
 // static_kernels.hip.
 //
 // Variants:
-//   x forward  "p<P>"            P row pairs per workgroup
-//   x inverse  "p<P>"
+//   x forward  "w<N>"            about N threads (N/T row pairs) per workgroup
+//              "w<N>-flat"       rows staged through LDS with workgroup-wide
+//                                coalesced loads and stores
+//   x inverse  "w<N>", "w<N>-flat"
+//              "w256-ldsmirror"  Hermitian extension formed in LDS
 //   column     "t<TX>"           one TX-bin tile per workgroup
 //              "t<TX>-pipe"      persistent workgroups; the next tile's loads
 //                                are issued before the current tile's FFT
@@ -52,14 +55,6 @@ constexpr int fit_pairs() {
   return p;
 }
 
-#ifndef FC_BATCH_XTGT
-#define FC_BATCH_XTGT 256
-#endif
-// row pairs per workgroup of the batched x passes
-template <int L>
-constexpr int batch_pairs() {
-  return fit_pairs<L, pairs_for<L, FC_BATCH_XTGT>()>();
-}
 
 // padded LDS row of the x passes: one float2 of padding per 8 keeps the
 // stride-8 and stride-Ns stores of the early stages off a single bank set
@@ -173,14 +168,6 @@ sk_xfwd(XPass2 a2) {
                       lds);
 }
 
-// batched: workgroup i serves tasks[i] = {argument record, row-pair group}
-template <int L, int PAIRS>
-__global__ void __launch_bounds__(PAIRS * threads_per_transform<L>())
-sb_xfwd(const XPassArgs* __restrict__ args, const int2* __restrict__ tasks) {
-  __shared__ float2 lds[PAIRS * XRow<L>::LP];
-  const int2 tk = tasks[blockIdx.x];
-  xfwd_body<L, PAIRS>(args[tk.x], tk.y, lds);
-}
 
 // ---------------------------------------------------------------------------
 // x pass inverse (C2R), two rows per complex FFT
@@ -326,14 +313,287 @@ sk_xinv(XPassArgs a) {
   xinv_body<L, PAIRS>(a, blockIdx.x, lds);
 }
 
+
+// ---------------------------------------------------------------------------
+// x passes with workgroup-wide ("flat") global access.  In the bodies above
+// the T threads of a row pair touch T consecutive elements of each of its
+// rows per instruction, so for short rows one wavefront instruction spans up
+// to 64/T rows and as many cache lines.  Here the workgroup's rows, which are
+// adjacent in memory, move between HBM and an LDS staging area as one run:
+// consecutive lanes take consecutive 16-byte (real rows) or 8-byte
+// (spectrum bins) pieces.  The staging area aliases the exchange rows.
+// ---------------------------------------------------------------------------
+template <int L>
+struct XStage {
+  static constexpr bool V4 = (L % 4) == 0;       // real rows split into float4s
+  static constexpr int SL = V4 ? L + 4 : L + 1;  // staging row stride (floats)
+  static constexpr int NB = L / 2 + 1;           // half-spectrum bins per row
+};
+
+// LDS floats of a flat x-pass workgroup of `pairs` row pairs: the largest of
+// the exchange rows, the real staging rows and the spectrum staging rows
+template <int L>
+constexpr int xflat_lds_floats(int pairs) {
+  const int ex = 2 * pairs * XRow<L>::LP;
+  const int st = 2 * pairs * XStage<L>::SL;
+  const int sc = 4 * pairs * XStage<L>::NB;
+  const int m = ex > st ? ex : st;
+  return m > sc ? m : sc;
+}
+
+// row pairs of a flat x-pass workgroup: about TGT threads, within 1024
+// threads and 64 KiB of LDS
+template <int L, int TGT>
+constexpr int flat_pairs() {
+  int p = pairs_for<L, TGT>();
+  while (p > 1 && (p * threads_per_transform<L>() > 1024 || xflat_lds_floats<L>(p) * 4 > 65536))
+    p /= 2;
+  return p;
+}
+
+#ifndef FC_BATCH_XTGT
+#define FC_BATCH_XTGT 256
+#endif
+#ifndef FC_BATCH_XFLAT
+#define FC_BATCH_XFLAT 1
+#endif
+// row pairs per workgroup of the batched x passes
+template <int L>
+constexpr int batch_pairs() {
+  return FC_BATCH_XFLAT ? flat_pairs<L, FC_BATCH_XTGT>() : fit_pairs<L, pairs_for<L, FC_BATCH_XTGT>()>();
+}
+
+// nr dense rows of L floats at g -> staging rows of stride SL
+template <int L, int NT, int MAXR>
+__device__ __forceinline__ void xflat_load_rows(const float* __restrict__ g, float* st, int nr) {
+  constexpr int SL = XStage<L>::SL;
+  if constexpr (XStage<L>::V4) {
+    if ((reinterpret_cast<uintptr_t>(g) & 15) == 0) {
+      constexpr int Q = L / 4, IT = (MAXR * Q + NT - 1) / NT;
+      const int n = nr * Q;
+      const float4* g4 = reinterpret_cast<const float4*>(g);
+      float4 r[IT];
+#pragma unroll
+      for (int j = 0; j < IT; ++j) {
+        const int i = threadIdx.x + j * NT;
+        r[j] = i < n ? g4[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+      }
+#pragma unroll
+      for (int j = 0; j < IT; ++j) {
+        const int i = threadIdx.x + j * NT;
+        if (i < n) {
+          const int row = i / Q;
+          *reinterpret_cast<float4*>(st + row * SL + 4 * (i - row * Q)) = r[j];
+        }
+      }
+      return;
+    }
+  }
+  constexpr int IT = (MAXR * L + NT - 1) / NT;
+  const int n = nr * L;
+  float r[IT];
+#pragma unroll
+  for (int j = 0; j < IT; ++j) {
+    const int i = threadIdx.x + j * NT;
+    r[j] = i < n ? g[i] : 0.f;
+  }
+#pragma unroll
+  for (int j = 0; j < IT; ++j) {
+    const int i = threadIdx.x + j * NT;
+    if (i < n) {
+      const int row = i / L;
+      st[row * SL + (i - row * L)] = r[j];
+    }
+  }
+}
+
+// staging rows of stride SL -> nr dense rows of L floats at g
+template <int L, int NT, int MAXR>
+__device__ __forceinline__ void xflat_store_rows(float* __restrict__ g, const float* st, int nr) {
+  constexpr int SL = XStage<L>::SL;
+  if constexpr (XStage<L>::V4) {
+    if ((reinterpret_cast<uintptr_t>(g) & 15) == 0) {
+      constexpr int Q = L / 4, IT = (MAXR * Q + NT - 1) / NT;
+      const int n = nr * Q;
+      float4* g4 = reinterpret_cast<float4*>(g);
+#pragma unroll
+      for (int j = 0; j < IT; ++j) {
+        const int i = threadIdx.x + j * NT;
+        if (i < n) {
+          const int row = i / Q;
+          g4[i] = *reinterpret_cast<const float4*>(st + row * SL + 4 * (i - row * Q));
+        }
+      }
+      return;
+    }
+  }
+  constexpr int IT = (MAXR * L + NT - 1) / NT;
+  const int n = nr * L;
+#pragma unroll
+  for (int j = 0; j < IT; ++j) {
+    const int i = threadIdx.x + j * NT;
+    if (i < n) {
+      const int row = i / L;
+      g[i] = st[row * SL + (i - row * L)];
+    }
+  }
+}
+
+// nr spectrum rows (NB bins each, row pitch `pitch`) -> dense staging rows
+template <int L, int NT, int MAXR>
+__device__ __forceinline__ void xflat_load_bins(const float2* __restrict__ g, long pitch,
+                                                float2* sc, int nr) {
+  constexpr int NB = XStage<L>::NB, IT = (MAXR * NB + NT - 1) / NT;
+  const int n = nr * NB;
+  float2 r[IT];
+#pragma unroll
+  for (int j = 0; j < IT; ++j) {
+    const int i = threadIdx.x + j * NT;
+    r[j] = make_float2(0.f, 0.f);
+    if (i < n) {
+      const int row = i / NB;
+      r[j] = g[row * pitch + (i - row * NB)];
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < IT; ++j) {
+    const int i = threadIdx.x + j * NT;
+    if (i < n) sc[i] = r[j];
+  }
+}
+
+// R2C of up to 2*PAIRS dense image rows starting at row 2*blk*PAIRS
+template <int L, int PAIRS>
+__device__ __forceinline__ void xfwd_flat_body(const XPassArgs& a, long blk, float2* lds) {
+  using P = PlanOf<L, false>;
+  constexpr int T = P::T, LP = XRow<L>::LP, SL = XStage<L>::SL, NB = XStage<L>::NB;
+  constexpr int NT = PAIRS * T, MAXR = 2 * PAIRS;
+  const long r0 = 2 * blk * PAIRS;
+  if (r0 >= a.nrows) return;  // whole workgroup
+  const int nr = a.nrows - r0 < MAXR ? (int)(a.nrows - r0) : MAXR;
+  float* st = reinterpret_cast<float*>(lds);
+  xflat_load_rows<L, NT, MAXR>(a.src + r0 * L, st, nr);
+  __syncthreads();
+  const int t = threadIdx.x % T, c = threadIdx.x / T;
+  const bool oka = 2 * c < nr, okb = 2 * c + 1 < nr;
+  float2 v[P::MAXE()];
+  sfft_for_inputs<L, false>(t, [&](int slot, int pos) {
+    v[slot] = make_float2(oka ? st[2 * c * SL + pos] : 0.f, okb ? st[(2 * c + 1) * SL + pos] : 0.f);
+  });
+  __syncthreads();
+  float2* row = lds + c * LP;
+  auto addr = [&](int pos) { return XRow<L>::at(pos); };
+  sfft_run<L, false, -1>(v, t, row, addr, a.fft.stw);
+  sfft_for_outputs<L, false>(t, [&](int slot, int pos) { row[addr(pos)] = v[slot]; });
+  __syncthreads();
+  // Hermitian split, bin-contiguous over the workgroup's rows
+  float2* dst = a.dst + r0 * a.dst_row;
+  constexpr int IT = (MAXR * NB + NT - 1) / NT;
+  const int n = nr * NB;
+#pragma unroll
+  for (int j = 0; j < IT; ++j) {
+    const int i = threadIdx.x + j * NT;
+    if (i < n) {
+      const int r = i / NB, kb = i - r * NB;
+      const float2* pr = lds + (r >> 1) * LP;
+      const float2 Z = pr[addr(kb)];
+      const float2 Zm = pr[addr(kb == 0 ? 0 : L - kb)];
+      dst[r * a.dst_row + kb] = (r & 1) ? make_float2(0.5f * (Z.y + Zm.y), 0.5f * (Zm.x - Z.x))
+                                        : make_float2(0.5f * (Z.x + Zm.x), 0.5f * (Z.y - Zm.y));
+    }
+  }
+}
+
+// C2R of up to 2*PAIRS spectrum rows into dense real rows
+template <int L, int PAIRS>
+__device__ __forceinline__ void xinv_flat_body(const XPassArgs& a, long blk, float2* lds) {
+  using P = PlanOf<L, false>;
+  constexpr int T = P::T, LP = XRow<L>::LP, SL = XStage<L>::SL, NB = XStage<L>::NB;
+  constexpr int NT = PAIRS * T, MAXR = 2 * PAIRS;
+  const long r0 = 2 * blk * PAIRS;
+  if (r0 >= a.nrows) return;  // whole workgroup
+  const int nr = a.nrows - r0 < MAXR ? (int)(a.nrows - r0) : MAXR;
+  xflat_load_bins<L, NT, MAXR>(a.csrc + r0 * a.src_row, a.src_row, lds, nr);
+  __syncthreads();
+  const int t = threadIdx.x % T, c = threadIdx.x / T;
+  const bool oka = 2 * c < nr, okb = 2 * c + 1 < nr;
+  const float2* sa = lds + 2 * c * NB;
+  const float2* sb = sa + NB;
+  float2 v[P::MAXE()];
+  // Z = A' + iB' over all L positions; position L - kb takes the conjugates
+  sfft_for_inputs<L, false>(t, [&](int slot, int pos) {
+    const bool mirror = pos >= NB;
+    const int kb = mirror ? L - pos : pos;
+    float2 A = oka ? sa[kb] : make_float2(0.f, 0.f);
+    float2 B = okb ? sb[kb] : make_float2(0.f, 0.f);
+    if (kb == 0 || 2 * kb == L) {
+      A.y = 0.f;
+      B.y = 0.f;
+    }
+    if (mirror) {
+      A.y = -A.y;
+      B.y = -B.y;
+    }
+    v[slot] = make_float2(A.x - B.y, A.y + B.x);
+  });
+  __syncthreads();
+  float2* row = lds + c * LP;
+  auto addr = [&](int pos) { return XRow<L>::at(pos); };
+  sfft_run<L, false, +1>(v, t, row, addr, a.fft.stw);
+  // sfft_run ends on a barrier after its last LDS read
+  float* st = reinterpret_cast<float*>(lds);
+  sfft_for_outputs<L, false>(t, [&](int slot, int pos) {
+    st[2 * c * SL + pos] = v[slot].x;
+    st[(2 * c + 1) * SL + pos] = v[slot].y;
+  });
+  __syncthreads();
+  xflat_store_rows<L, NT, MAXR>(a.rdst + r0 * L, st, nr);
+}
+
+template <int L, int PAIRS>
+__global__ void __launch_bounds__(PAIRS * threads_per_transform<L>())
+sk_xfwd_flat(XPass2 a2) {
+  static_assert(xflat_lds_floats<L>(PAIRS) * 4 <= 65536, "flat x pass LDS");
+  __shared__ float2 lds[xflat_lds_floats<L>(PAIRS) / 2];
+  if ((long)blockIdx.x >= a2.k_blocks)
+    xfwd_flat_body<L, PAIRS>(a2.m, (long)blockIdx.x - a2.k_blocks, lds);
+  else
+    xfwd_body<L, PAIRS>(a2.k, blockIdx.x, lds);
+}
+
+template <int L, int PAIRS>
+__global__ void __launch_bounds__(PAIRS * threads_per_transform<L>())
+sk_xinv_flat(XPassArgs a) {
+  static_assert(xflat_lds_floats<L>(PAIRS) * 4 <= 65536, "flat x pass LDS");
+  __shared__ float2 lds[xflat_lds_floats<L>(PAIRS) / 2];
+  xinv_flat_body<L, PAIRS>(a, blockIdx.x, lds);
+}
+
+// batched x passes: workgroup i serves tasks[i] = {argument record, row-pair
+// group}.  Image rows (dense, unshifted) take the flat bodies; the compact
+// kernel's rows the per-pair body.
+template <int L, int PAIRS>
+__global__ void __launch_bounds__(PAIRS * threads_per_transform<L>())
+sb_xfwd(const XPassArgs* __restrict__ args, const int2* __restrict__ tasks) {
+  __shared__ float2 lds[xflat_lds_floats<L>(PAIRS) / 2];
+  const int2 tk = tasks[blockIdx.x];
+  const XPassArgs& a = args[tk.x];
+  if (FC_BATCH_XFLAT && a.src_row == L && a.klen == L && a.kcenter == 0)
+    xfwd_flat_body<L, PAIRS>(a, tk.y, lds);
+  else
+    xfwd_body<L, PAIRS>(a, tk.y, lds);
+}
+
 template <int L, int PAIRS>
 __global__ void __launch_bounds__(PAIRS * threads_per_transform<L>())
 sb_xinv(const XPassArgs* __restrict__ args, const int2* __restrict__ tasks) {
-  __shared__ float2 lds[PAIRS * XRow<L>::LP];
+  __shared__ float2 lds[xflat_lds_floats<L>(PAIRS) / 2];
   const int2 tk = tasks[blockIdx.x];
-  // Hermitian extension in LDS: measured ~1 % faster over the 64-block
-  // menu workload than reading each bin twice
-  xinv_lds_body<L, PAIRS>(args[tk.x], tk.y, lds);
+  const XPassArgs& a = args[tk.x];
+  if (FC_BATCH_XFLAT && a.dst_row == L)
+    xinv_flat_body<L, PAIRS>(a, tk.y, lds);
+  else
+    xinv_lds_body<L, PAIRS>(a, tk.y, lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -771,6 +1031,29 @@ bool sk_launch_xinv_lds(const XPassArgs& a, hipStream_t st) {
   return true;
 }
 
+// flat variants need dense real rows (row stride L)
+template <int L, int TGT>
+bool sk_launch_xfwd_flat(const XPass2& a, hipStream_t st) {
+  if (a.m.nrows > 0 && a.m.src_row != L) return false;
+  constexpr int P = flat_pairs<L, TGT>();
+  XPass2 b = a;
+  b.m_blocks = ((a.m.nrows + 1) / 2 + P - 1) / P;
+  b.k_blocks = ((a.k.nrows + 1) / 2 + P - 1) / P;
+  hipLaunchKernelGGL((sk_xfwd_flat<L, P>), dim3((unsigned)(b.m_blocks + b.k_blocks)),
+                     dim3(P * threads_per_transform<L>()), 0, st, b);
+  return true;
+}
+
+template <int L, int TGT>
+bool sk_launch_xinv_flat(const XPassArgs& a, hipStream_t st) {
+  if (a.dst_row != L) return false;
+  constexpr int P = flat_pairs<L, TGT>();
+  const long blocks = ((a.nrows + 1) / 2 + P - 1) / P;
+  hipLaunchKernelGGL((sk_xinv_flat<L, P>), dim3((unsigned)blocks),
+                     dim3(P * threads_per_transform<L>()), 0, st, a);
+  return true;
+}
+
 template <int L, int TX>
 Col2 col2_tiles(const Col2& a) {
   Col2 b = a;
@@ -886,11 +1169,15 @@ StaticEntry make_static_entry() {
   StaticEntry e;
   e.xfwd = {{"w256", &sk_launch_xfwd<L, 256>},
             {"w128", &sk_launch_xfwd<L, 128>},
-            {"w64", &sk_launch_xfwd<L, 64>}};
+            {"w64", &sk_launch_xfwd<L, 64>},
+            {"w256-flat", &sk_launch_xfwd_flat<L, 256>},
+            {"w128-flat", &sk_launch_xfwd_flat<L, 128>}};
   e.xinv = {{"w256", &sk_launch_xinv<L, 256>},
             {"w128", &sk_launch_xinv<L, 128>},
             {"w64", &sk_launch_xinv<L, 64>},
-            {"w256-ldsmirror", &sk_launch_xinv_lds<L, 256>}};
+            {"w256-ldsmirror", &sk_launch_xinv_lds<L, 256>},
+            {"w256-flat", &sk_launch_xinv_flat<L, 256>},
+            {"w128-flat", &sk_launch_xinv_flat<L, 128>}};
   e.col = {{"t16", &sk_launch_col<L, 16>},
            {"t16-pipe", &sk_launch_col_pipe<L, 16>},
            {"t8-pipe", &sk_launch_col_pipe<L, 8>},
@@ -903,6 +1190,16 @@ StaticEntry make_static_entry() {
               {"t8-fft", &sk_launch_zfused_fft<L, 8>},
               {"t32-fft", &sk_launch_zfused_fft<L, 32>},
               {"t64-fft", &sk_launch_zfused_fft<L, 64>}};
+  // defaults (measured over (512, 1024, L) volumes): the flat x inverse at
+  // every length; the flat x forward where rows are short or not a whole
+  // number of float4s, the per-pair forward elsewhere
+  auto index_of = [](const auto& vs, const char* name) {
+    for (size_t i = 0; i < vs.size(); ++i)
+      if (__builtin_strcmp(vs[i].name, name) == 0) return (int)i;
+    return 0;
+  };
+  e.selected[kSxInv] = index_of(e.xinv, "w256-flat");
+  if (L < 128 || L % 4 != 0) e.selected[kSxFwd] = index_of(e.xfwd, "w256-flat");
   e.xfwd_vpad = &sk_launch_xfwd_vpad<L, 64>;
   e.xinv_crop = &sk_launch_xinv_crop<L, 64>;
   e.batch.pairs = batch_pairs<L>();

@@ -84,6 +84,7 @@ def test_every_variant(fc, oracle, L):
     lib = fc.lib()
     cases = {0: [(6, 10, L)], 1: [(6, 10, L)], 2: [(6, L, 24)],
              3: [(L, 10, 24), (L, 6, 18)]}
+    prev = {p: lib.fcamd_selected_static_variant(p, L) for p in cases}
     try:
         for p, shapes in cases.items():
             n = lib.fcamd_static_variant_count(p, L)
@@ -96,10 +97,47 @@ def test_every_variant(fc, oracle, L):
                     else:  # kz beyond the direct-DFT limit -> fallback variant
                         kshape = (min(L, 61 if L >= 128 else L - 1), 5, 3)
                     _check(fc, oracle, shape, kshape, 10 * p + v)
-            lib.fcamd_select_static_variant(p, L, 0)
     finally:
         for p in cases:
-            lib.fcamd_select_static_variant(p, L, 0)
+            lib.fcamd_select_static_variant(p, L, prev[p])
+
+
+@pytest.mark.parametrize("L", [64, 105, 128, 210, 240, 512])
+def test_flat_x_variants_many_workgroups(fc, oracle, L):
+    """The flat x variants (workgroup-wide staging through LDS) over many
+    workgroups with a ragged last one, with 16-byte-aligned tensors (float4
+    path where L % 4 == 0) and with image and output one float off that
+    alignment (scalar path)."""
+    import torch
+    from fourierconvolutioncudalib_amd import synth
+    lib = fc.lib()
+    shape, kshape = (7, 41, L), (5, 7, 3)
+    img = synth.uniform(shape, 31)
+    k = synth.random_psf(kshape, 5)
+    ref = oracle.numpy_fft_conv(img, k)
+    tk = torch.from_numpy(k).cuda()
+    n = img.size
+    prev = {p: lib.fcamd_selected_static_variant(p, L) for p in (0, 1)}
+    try:
+        for name in ("w256-flat", "w128-flat"):
+            for p in (0, 1):
+                names = [lib.fcamd_static_variant_name(p, L, v).decode()
+                         for v in range(lib.fcamd_static_variant_count(p, L))]
+                assert lib.fcamd_select_static_variant(p, L, names.index(name)) == 0
+            out = fc.convolve(img, k)
+            assert oracle.rel_l2(out, ref) < TOL, (name, "aligned")
+            buf = torch.zeros(n + 1, device="cuda")
+            ti = buf[1:].view(shape)
+            ti.copy_(torch.from_numpy(img).cuda())
+            obuf = torch.zeros(n + 1, device="cuda")
+            to = obuf[1:].view(shape)
+            assert ti.data_ptr() % 16 != 0 and to.data_ptr() % 16 != 0
+            fc.convolve3d(ti, tk, out=to)
+            assert oracle.rel_l2(to.cpu().numpy(), ref) < TOL, (name, "unaligned")
+            assert obuf[0].item() == 0.0  # nothing written before the output
+    finally:
+        for p in (0, 1):
+            lib.fcamd_select_static_variant(p, L, prev[p])
 
 
 def test_autotune_keeps_results_and_writes_cache(fc, oracle, tmp_path, monkeypatch):
@@ -109,6 +147,9 @@ def test_autotune_keeps_results_and_writes_cache(fc, oracle, tmp_path, monkeypat
     cache = tmp_path / "tune.txt"
     monkeypatch.setenv("FCAMD_TUNE_CACHE", str(cache))
     shape, kshape = (64, 96, 128), (9, 7, 5)
+    lib = fc.lib()
+    tuned = [(0, 128), (1, 128), (2, 96), (3, 64)]
+    prev = {pl: lib.fcamd_selected_static_variant(*pl) for pl in tuned}
     fc.autotune(shape, kshape, reps=2)
     sel = fc.selected_variants(shape)
     assert set(sel) == {"x_forward", "x_inverse", "y", "z"}
@@ -117,7 +158,6 @@ def test_autotune_keeps_results_and_writes_cache(fc, oracle, tmp_path, monkeypat
     img = synth.uniform(shape, 17)
     k = synth.random_psf(kshape, 18)
     assert oracle.rel_l2(fc.convolve(img, k), oracle.numpy_fft_conv(img, k)) < 1e-5
-    # restore defaults for the tests that follow
-    lib = fc.lib()
-    for p, L in [(0, 128), (1, 128), (2, 96), (3, 64)]:
-        lib.fcamd_select_static_variant(p, L, 0)
+    # restore the defaults for the tests that follow
+    for (p, L), v in prev.items():
+        lib.fcamd_select_static_variant(p, L, v)
