@@ -226,11 +226,19 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--only", choices=["batched", "rl"],
+                    help="run one side comparison alone and print its JSON (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(the path's sharding is described in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(every convolution here runs on a single device)", file=sys.stderr)
         sys.exit(2)
+    if args.only:
+        import torch
+        import fourierconvolutioncudalib_amd as fc
+        side = batched_comparison if args.only == "batched" else rl_comparison
+        print(json.dumps({args.only: side(fc, torch)}))
+        return
 
     import numpy as np
     import torch

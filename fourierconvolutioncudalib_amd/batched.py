@@ -1,10 +1,12 @@
 """Batched convolution of many uneven blocks (torch CUDA tensors).
 
-Each block has its own image shape and kernel.  The library groups the
-blocks of every pass by transform length, launches one compile-time kernel
-per length (length groups on forked streams) and can capture the whole pass
-sequence in a hipGraph that is replayed on later calls.  Buffers are bound
-when the plan is created.
+Each block has its own image shape and kernel.  By default the library
+deals the blocks to a few streams ("lanes", balanced by estimated cost),
+each lane running its blocks' pass sequences in its own workspace slot;
+FCAMD_BATCH_LANES=0 selects the length-group schedule instead (one
+compile-time kernel launch per transform length and pass, length groups on
+forked streams).  Either sequence can be captured in a hipGraph that is
+replayed on later calls.  Buffers are bound when the plan is created.
 """
 from __future__ import annotations
 

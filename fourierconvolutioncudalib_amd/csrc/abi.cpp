@@ -175,7 +175,7 @@ void convolution3DfftCUDAInPlace(imageType* im, int* imDim, imageType* kernel, i
 // z-fastest (imDim[2] contiguous), so {imDim[2], imDim[1], imDim[0]} is the
 // {x, y, z} of the in-place entry point.  The reference computes this
 // in-place on an unpadded R2C buffer, valid only under the long-removed
-// cuFFT native layout (:298,358-361); this build returns the convolution.
+// cuFFT native layout (:298,358-361); here the call returns the convolution.
 imageType* convolution3DfftCUDA(imageType* im, int* imDim, imageType* kernel, int* kernelDim,
                                 int devCUDA) {
   float* result = nullptr;

@@ -1,23 +1,28 @@
-// batch.cpp -- many convolutions of uneven shapes as one pass sequence.
+// batch.cpp -- many convolutions of uneven shapes in one plan.
 //
 // Every block (image, kernel, output; its own shape) goes through the same
-// five passes as a single convolution.  For each pass the blocks are
-// grouped by the transform length of that pass's axis; each group is one
-// launch of the compile-time kernel for that length over a device task
-// table {argument record, tile}, so a pass costs one launch per distinct
-// length instead of one per block.  Groups of one pass are independent and
-// run on forked streams; the pass boundary joins them.  The whole sequence
-// can be captured once into a hipGraph and replayed.
+// five passes as a single convolution.  Two schedules:
 //
-// Blocks are executed in waves: consecutive blocks whose workspaces add up
-// to at most a byte budget form one wave, every wave runs all passes before
-// the next starts, and all waves share one workspace region.  With a budget
-// below the device's last-level (Infinity) cache, a wave's spectra stay in
-// that cache from pass to pass instead of going through HBM, and the
-// workspace shrinks to the largest wave.
+// Lanes (default).  The blocks are dealt to a few streams ("lanes"),
+// largest estimated cost first, each to the lane with the least work so
+// far; a lane runs its blocks' pass sequences one after another in its own
+// workspace slot.  One block's spectrum (tens of MB) stays in the 256 MB
+// Infinity Cache from pass to pass, and the lanes' launches overlap each
+// other's ramp-up and drain, so neither HBM round trips between passes nor
+// per-launch fixed costs are paid in full.  The workspace is one slot per
+// lane.
 //
-// Blocks whose length has no compile-time kernel fall back to the
-// runtime-length kernels, one launch per block.
+// Length groups.  For each pass the blocks are grouped by the transform
+// length of that pass's axis; each group is one launch of the compile-time
+// kernel for that length over a device task table {argument record, tile},
+// so a pass costs one launch per distinct length instead of one per block.
+// Groups of one pass run on forked streams; the pass boundary joins them.
+// Blocks run in waves: consecutive blocks whose workspaces add up to at
+// most a byte budget run all passes before the next wave starts, sharing
+// one workspace region.  Blocks whose length has no compile-time kernel
+// fall back to the runtime-length kernels, one launch per block.
+//
+// Either sequence can be captured once into a hipGraph and replayed.
 #include <algorithm>
 #include <cstdlib>
 #include <map>
@@ -36,6 +41,15 @@ size_t align256(size_t b) { return (b + 255) / 256 * 256; }
 
 // Default wave budget (bytes of block workspace per wave; 0 = one wave).
 constexpr size_t kDefaultWaveBytes = 0;
+
+// Default number of lanes (0 = the length-group schedule).  Four matches
+// the hardware queues a process gets by default; more lanes share them.
+constexpr int kDefaultLanes = 4;
+
+// A block's estimated cost for lane balancing, in voxel units: its voxels
+// plus a constant for the fixed cost of its five launches (~5 x 7 us, the
+// time the passes take to move ~3.5 M voxels).
+constexpr long kLaunchCostVoxels = 3500000;
 
 struct Block {
   ConvShape s;
@@ -109,6 +123,41 @@ size_t default_wave_budget() {
 
 size_t block_workspace(const ConvShape& s) { return align256(workspace_layout(s).total()); }
 
+// Lane count: FCAMD_BATCH_LANES when set (0 = length groups), else the
+// default above.
+int default_lanes() {
+  if (const char* v = std::getenv("FCAMD_BATCH_LANES")) return std::max(0, std::atoi(v));
+  return kDefaultLanes;
+}
+
+// Lane assignment (longest processing time first): blocks by decreasing
+// estimated cost, each to the lane with the least accumulated cost; ties
+// go to the lowest lane.  Returns the blocks of every non-empty lane in
+// execution order and each lane's workspace slot (its largest block).
+std::vector<std::vector<int>> lane_plan(const std::vector<ConvShape>& shapes, int nlanes,
+                                        std::vector<size_t>* slot_bytes) {
+  const int n = (int)shapes.size();
+  nlanes = std::max(1, std::min(nlanes, n));
+  std::vector<int> order(n);
+  std::vector<long> cost(n);
+  for (int b = 0; b < n; ++b) {
+    order[b] = b;
+    cost[b] = shapes[b].voxels() + kLaunchCostVoxels;
+  }
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+  std::vector<std::vector<int>> lanes(nlanes);
+  std::vector<long> load(nlanes, 0);
+  std::vector<size_t> slot(nlanes, 0);
+  for (int b : order) {
+    const int l = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    lanes[l].push_back(b);
+    load[l] += cost[b];
+    slot[l] = std::max(slot[l], block_workspace(shapes[b]));
+  }
+  if (slot_bytes) *slot_bytes = slot;
+  return lanes;
+}
+
 ConvShape block_shape(const int* imDims, const int* kernelDims, int b) {
   ConvShape s{};
   s.nx = imDims[3 * b];
@@ -127,7 +176,7 @@ class BatchPlan {
  public:
   BatchPlan(int n, const int* imDims, const int* kernelDims, float* const* outs,
             const float* const* imgs, const float* const* kers, int op, void* ws, size_t ws_bytes,
-            size_t wave_budget);
+            size_t wave_budget, int nlanes);
   ~BatchPlan();
   void enqueue(hipStream_t st);
   void graph_launch(hipStream_t st);
@@ -144,6 +193,7 @@ class BatchPlan {
   void fork(hipStream_t st, size_t ngroups, F launch_group);
   void build_wave(Wave& w, int first, int last);
   void enqueue_wave(const Wave& w, hipStream_t st);
+  void enqueue_lanes(hipStream_t st);
 
   int dev_ = 0;
   bool conj_ = false;
@@ -152,6 +202,9 @@ class BatchPlan {
   void* ws_ = nullptr;
   bool own_ws_ = false;
   std::vector<Wave> waves_;
+  // lane schedule: blocks of every lane, and each lane's workspace slot
+  std::vector<std::vector<int>> lanes_;
+  std::vector<size_t> lane_off_;
   std::vector<hipStream_t> side_;
   std::vector<hipEvent_t> ev_;
   hipGraphExec_t exec_ = nullptr;
@@ -161,7 +214,7 @@ class BatchPlan {
 
 BatchPlan::BatchPlan(int n, const int* imDims, const int* kernelDims, float* const* outs,
                      const float* const* imgs, const float* const* kers, int op, void* ws,
-                     size_t ws_bytes, size_t wave_budget) {
+                     size_t ws_bytes, size_t wave_budget, int nlanes) {
   FC_REQUIRE(n >= 1, kInvalidArgument, "empty batch");
   FC_REQUIRE(imDims && kernelDims && outs && imgs && kers, kInvalidArgument, "null array");
   FC_REQUIRE(op == FCAMD_CONVOLVE || op == FCAMD_CORRELATE, kInvalidArgument, "unknown op");
@@ -179,13 +232,47 @@ BatchPlan::BatchPlan(int n, const int* imDims, const int* kernelDims, float* con
     sizes.push_back(block_workspace(bl.s));
     blocks_.push_back(bl);
   }
-  const std::vector<int> starts = wave_starts(sizes, wave_budget, &ws_need_);
+  std::vector<int> starts;
+  if (nlanes > 0) {
+    std::vector<ConvShape> shapes;
+    for (const Block& bl : blocks_) shapes.push_back(bl.s);
+    std::vector<size_t> slot;
+    lanes_ = lane_plan(shapes, nlanes, &slot);
+    ws_need_ = 0;
+    for (size_t l = 0; l < lanes_.size(); ++l) {
+      lane_off_.push_back(ws_need_);
+      ws_need_ += slot[l];
+    }
+  } else {
+    starts = wave_starts(sizes, wave_budget, &ws_need_);
+  }
   if (ws) {
     FC_REQUIRE(ws_bytes >= ws_need_, kInvalidArgument, "batch workspace too small");
     ws_ = ws;
   } else {
     FC_HIP_CHECK(hipMalloc(&ws_, ws_need_));
     own_ws_ = true;
+  }
+  driver_init();
+  // create every twiddle table now: nothing may allocate or copy while a
+  // graph is being captured
+  for (const Block& bl : blocks_) {
+    (void)fft_desc(bl.s.nx);
+    (void)fft_desc(bl.s.ny);
+    (void)fft_desc(bl.s.nz);
+  }
+  if (nlanes > 0) {
+    for (size_t i = 0; i + 1 < lanes_.size(); ++i) {
+      hipStream_t st = nullptr;
+      FC_HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+      side_.push_back(st);
+    }
+    for (size_t i = 0; i < lanes_.size(); ++i) {
+      hipEvent_t e = nullptr;
+      FC_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      ev_.push_back(e);
+    }
+    return;
   }
   // every wave lays its blocks out from the start of the workspace
   for (size_t w = 0; w < starts.size(); ++w) {
@@ -199,14 +286,6 @@ BatchPlan::BatchPlan(int n, const int* imDims, const int* kernelDims, float* con
       blocks_[b].Kxy = reinterpret_cast<float2*>(base + wl.spectrum + wl.krows);
       off += sizes[b];
     }
-  }
-  driver_init();
-  // create every twiddle table now: nothing may allocate or copy while a
-  // graph is being captured
-  for (const Block& bl : blocks_) {
-    (void)fft_desc(bl.s.nx);
-    (void)fft_desc(bl.s.ny);
-    (void)fft_desc(bl.s.nz);
   }
   waves_.resize(starts.size());
   size_t nside = 1;
@@ -449,10 +528,40 @@ void BatchPlan::enqueue_wave(const Wave& w, hipStream_t st) {
   for (int b : w.generic_x) pass_x_inverse(blocks_[b].s, blocks_[b].S, blocks_[b].out, st);
 }
 
+// Lane 0 runs on `st`, the others on side streams forked from and joined
+// back into `st`.  Blocks are enqueued round-robin over the lanes, so in
+// eager mode every lane has work queued from the start.
+void BatchPlan::enqueue_lanes(hipStream_t st) {
+  const size_t nl = lanes_.size();
+  if (nl > 1) {
+    FC_HIP_CHECK(hipEventRecord(ev_[0], st));
+    for (size_t i = 1; i < nl; ++i) FC_HIP_CHECK(hipStreamWaitEvent(side_[i - 1], ev_[0], 0));
+  }
+  const Op op = conj_ ? Op::kCorrelate : Op::kConvolve;
+  size_t depth = 0;
+  for (const auto& lane : lanes_) depth = std::max(depth, lane.size());
+  for (size_t j = 0; j < depth; ++j) {
+    for (size_t i = 0; i < nl; ++i) {
+      if (j >= lanes_[i].size()) continue;
+      const Block& bl = blocks_[lanes_[i][j]];
+      enqueue_convolution(bl.s, bl.out, bl.img, bl.ker, op, static_cast<char*>(ws_) + lane_off_[i],
+                          i == 0 ? st : side_[i - 1]);
+    }
+  }
+  for (size_t i = 1; i < nl; ++i) {
+    FC_HIP_CHECK(hipEventRecord(ev_[i], side_[i - 1]));
+    FC_HIP_CHECK(hipStreamWaitEvent(st, ev_[i], 0));
+  }
+}
+
 void BatchPlan::enqueue(hipStream_t st) {
-  // waves share the workspace: stream order on `st` (every fork joins back
-  // into it) keeps wave w+1 from starting before wave w has finished
-  for (const Wave& w : waves_) enqueue_wave(w, st);
+  if (!lanes_.empty()) {
+    enqueue_lanes(st);
+  } else {
+    // waves share the workspace: stream order on `st` (every fork joins
+    // back into it) keeps wave w+1 from starting before wave w has finished
+    for (const Wave& w : waves_) enqueue_wave(w, st);
+  }
   FC_HIP_CHECK(hipGetLastError());
 }
 
@@ -510,14 +619,23 @@ long long fcamd_batch_workspace_bytes(int nblocks, const int* imDims, const int*
   (void)batch_guarded([&] {
     FC_REQUIRE(nblocks >= 1 && imDims && kernelDims, kInvalidArgument, "bad batch description");
     std::vector<size_t> sizes;
+    std::vector<ConvShape> shapes;
     for (int b = 0; b < nblocks; ++b) {
       const ConvShape s = block_shape(imDims, kernelDims, b);
       validate_shape(s);
       sizes.push_back(block_workspace(s));
+      shapes.push_back(s);
     }
-    size_t largest = 0;
-    (void)wave_starts(sizes, default_wave_budget(), &largest);
-    r = (long long)largest;
+    const int nlanes = default_lanes();
+    size_t need = 0;
+    if (nlanes > 0) {
+      std::vector<size_t> slot;
+      (void)lane_plan(shapes, nlanes, &slot);
+      for (size_t b : slot) need += b;
+    } else {
+      (void)wave_starts(sizes, default_wave_budget(), &need);
+    }
+    r = (long long)need;
   });
   return r;
 }
@@ -528,7 +646,8 @@ void* fcamd_batch_create(int nblocks, const int* imDims, const int* kernelDims, 
   BatchPlan* p = nullptr;
   (void)batch_guarded([&] {
     p = new BatchPlan(nblocks, imDims, kernelDims, d_outs, d_images, d_kernels, op, d_workspace,
-                      (size_t)(workspace_bytes < 0 ? 0 : workspace_bytes), default_wave_budget());
+                      (size_t)(workspace_bytes < 0 ? 0 : workspace_bytes), default_wave_budget(),
+                      default_lanes());
   });
   return p;
 }

@@ -3,7 +3,7 @@
  *
  * Replaces src/FourierConvolutionCUDALib_Export.h (reference), which CMake's
  * GenerateExportHeader writes for MSVC (src/CMakeLists.txt:17-25).  On Linux
- * (the only platform this build targets) exported symbols need no
+ * (the only platform this library targets) exported symbols need no
  * decoration beyond default visibility, so the same macro names are kept
  * and expand to a visibility attribute; FourierConvolutionCUDALib_BUILT_AS_STATIC
  * keeps its meaning for the static archive.

@@ -112,17 +112,22 @@ FCAMD_API void fcamd_conv_plan_destroy(void* plan);
 /* Batched uneven blocks (no counterpart in the reference): `nblocks`
  * convolutions, block b with dims imDims[3b..3b+2] ({x,y,z}), kernel dims
  * kernelDims[3b..3b+2], device buffers d_images[b], d_kernels[b] and
- * d_outs[b] (may alias the image).  The plan groups the blocks of every pass
- * by transform length (one launch per length, length groups on forked
- * streams); fcamd_batch_execute enqueues the sequence on `stream`,
- * fcamd_batch_graph_launch captures it into a hipGraph on the first call and
- * replays that graph afterwards.  The buffers are bound at creation.
- * Blocks run in waves of consecutive blocks whose workspaces add up to at
- * most FCAMD_BATCH_WAVE_MB MiB (environment, read at creation; unset = the
- * library default, 0 = a single wave); waves share one workspace region.
+ * d_outs[b] (may alias the image).  fcamd_batch_execute enqueues the
+ * sequence on `stream`, fcamd_batch_graph_launch captures it into a hipGraph
+ * on the first call and replays that graph afterwards.  The buffers are
+ * bound at creation.  Two schedules, chosen at creation by the environment:
+ *   FCAMD_BATCH_LANES=N (unset = 4): the blocks are dealt to N streams
+ *     ("lanes"), largest estimated cost first to the least-loaded lane;
+ *     each lane runs its blocks' pass sequences one after another in its
+ *     own workspace slot (the workspace is one slot per lane).
+ *   FCAMD_BATCH_LANES=0: the blocks of every pass are grouped by transform
+ *     length (one launch per length, length groups on forked streams), in
+ *     waves of consecutive blocks whose workspaces add up to at most
+ *     FCAMD_BATCH_WAVE_MB MiB (unset or 0 = a single wave); waves share one
+ *     workspace region.
  * d_workspace may be NULL (the plan allocates it) or hold
- * fcamd_batch_workspace_bytes() bytes (the largest wave's workspace under
- * the same setting).  create returns NULL on error. */
+ * fcamd_batch_workspace_bytes() bytes (computed under the same settings).
+ * create returns NULL on error. */
 FCAMD_API long long fcamd_batch_workspace_bytes(int nblocks, const int* imDims,
                                                 const int* kernelDims);
 FCAMD_API void* fcamd_batch_create(int nblocks, const int* imDims, const int* kernelDims,

@@ -1,6 +1,8 @@
 """Batched uneven blocks: every block against the FP64 oracle and against
 the plain per-block call, eager and through hipGraph replays (each replay
-checked, with eager work and synchronisation between replays)."""
+checked, with eager work and synchronisation between replays), under both
+schedules (lanes: per-block pass sequences dealt to a few streams; groups:
+one launch per transform length and pass)."""
 import numpy as np
 import pytest
 
@@ -26,7 +28,13 @@ SHAPES = [((64, 80, 96), (9, 11, 13)), ((105, 64, 112), (21, 9, 15)),
           ((160, 224, 240), (11, 13, 17))]
 
 
-def test_batched_matches_oracle_and_plain_calls(fc, oracle):
+@pytest.fixture(params=["lanes", "groups"])
+def schedule(request, monkeypatch):
+    monkeypatch.setenv("FCAMD_BATCH_LANES", "4" if request.param == "lanes" else "0")
+    return request.param
+
+
+def test_batched_matches_oracle_and_plain_calls(fc, oracle, schedule):
     import torch
     from fourierconvolutioncudalib_amd.batched import BatchedConvolution
     imgs, ks = _blocks(300, SHAPES)
@@ -41,7 +49,7 @@ def test_batched_matches_oracle_and_plain_calls(fc, oracle):
     plan.close()
 
 
-def test_batched_graph_replays(fc, oracle):
+def test_batched_graph_replays(fc, oracle, schedule):
     import torch
     from fourierconvolutioncudalib_amd.batched import BatchedConvolution
     shapes = SHAPES[:6]
@@ -63,7 +71,7 @@ def test_batched_graph_replays(fc, oracle):
     plan.close()
 
 
-def test_uneven_64_block_workload(fc, oracle):
+def test_uneven_64_block_workload(fc, oracle, schedule):
     """The 64-block menu workload (2/3/5/7-smooth extents, odd 9-41 kernels
     with random sigma and rotation): batched vs 64 plain calls for every
     block, FP64 oracle for the smaller blocks."""
@@ -104,6 +112,7 @@ def test_batched_waves(fc, oracle, monkeypatch, wave_mb):
     kdims = [d for k in ks for d in k.shape[::-1]]
     n = len(imgs)
     lib = fc.lib()
+    monkeypatch.setenv("FCAMD_BATCH_LANES", "0")  # waves belong to the group schedule
     one_wave = lib.fcamd_batch_workspace_bytes(n, (ctypes.c_int * (3 * n))(*dims),
                                                (ctypes.c_int * (3 * n))(*kdims))
     monkeypatch.setenv("FCAMD_BATCH_WAVE_MB", str(wave_mb))
@@ -116,6 +125,42 @@ def test_batched_waves(fc, oracle, monkeypatch, wave_mb):
     for e, p in zip(eager, plain):
         assert oracle.rel_l2(e, p) < 1e-6
     for rep in range(3):
+        got = [o.clone() for o in plan.run(graph=True)]
+        torch.cuda.synchronize()
+        for g, p in zip(got, plain):
+            assert oracle.rel_l2(g.cpu().numpy(), p) < 1e-6, rep
+    plan.close()
+
+
+@pytest.mark.parametrize("lanes", [1, 3, 16])
+def test_batched_lanes(fc, oracle, monkeypatch, lanes):
+    """The lane schedule with one lane, a few lanes and more lanes than
+    blocks: its workspace (one slot per lane) is at most the group
+    schedule's, and the results equal the plain calls, eager and over
+    repeated graph replays with eager work between them."""
+    import ctypes
+    import torch
+    from fourierconvolutioncudalib_amd.batched import BatchedConvolution
+    imgs, ks = _blocks(600, SHAPES)
+    ti = [torch.from_numpy(a).cuda() for a in imgs]
+    tk = [torch.from_numpy(k).cuda() for k in ks]
+    n = len(imgs)
+    dims = (ctypes.c_int * (3 * n))(*[d for a in imgs for d in a.shape[::-1]])
+    kdims = (ctypes.c_int * (3 * n))(*[d for k in ks for d in k.shape[::-1]])
+    lib = fc.lib()
+    monkeypatch.setenv("FCAMD_BATCH_LANES", "0")
+    grouped = lib.fcamd_batch_workspace_bytes(n, dims, kdims)
+    monkeypatch.setenv("FCAMD_BATCH_LANES", str(lanes))
+    laned = lib.fcamd_batch_workspace_bytes(n, dims, kdims)
+    assert 0 < laned <= grouped
+    plan = BatchedConvolution(ti, tk)
+    plain = [fc.convolve3d(a, k).cpu().numpy() for a, k in zip(ti, tk)]
+    for e, p in zip([o.cpu().numpy() for o in plan.run(graph=False)], plain):
+        assert oracle.rel_l2(e, p) < 1e-6
+    for rep in range(3):
+        for o in plan.outs:
+            o.fill_(float("nan"))
+        torch.cuda.synchronize()
         got = [o.clone() for o in plan.run(graph=True)]
         torch.cuda.synchronize()
         for g, p in zip(got, plain):
