@@ -69,6 +69,37 @@ def test_workspace_bytes(fc):
         fc.workspace_bytes((8, 8, 8), (9, 3, 3))
 
 
+def test_batch_workspace_lanes(fc, monkeypatch):
+    """Batch workspace under each schedule (host arithmetic only): one slot
+    per lane, the slot being the lane's largest block; blocks dealt largest
+    estimated cost first to the least-loaded lane.  With 0 lanes (length
+    groups, one wave) every block has its own region."""
+    def a256(b):
+        return (b + 255) // 256 * 256
+    shapes = [((64, 64, 64), (5, 5, 5)), ((32, 32, 32), (3, 3, 3)), ((128, 64, 64), (5, 5, 5)),
+              ((16, 16, 16), (3, 3, 3))]
+    ws = [fc.workspace_bytes(s, k) for s, k in shapes]
+    n = len(shapes)
+    dims = (ctypes.c_int * (3 * n))(*[d for s, _ in shapes for d in s[::-1]])
+    kdims = (ctypes.c_int * (3 * n))(*[d for _, k in shapes for d in k[::-1]])
+    lib = fc.lib()
+
+    def need(lanes):
+        monkeypatch.setenv("FCAMD_BATCH_LANES", str(lanes))
+        return lib.fcamd_batch_workspace_bytes(n, dims, kdims)
+
+    assert all(w == a256(w) for w in ws)
+    assert need(0) == sum(ws)
+    assert need(1) == max(ws)
+    assert need(n) == sum(ws) and need(3 * n) == sum(ws)
+    # two lanes, costs voxels + 3.5 M (4.02 M, 3.76 M, 3.53 M, 3.50 M for
+    # blocks 2, 0, 1, 3): 2 -> lane 0, 0 -> lane 1, 1 -> lane 1, 3 -> lane 0;
+    # slots ws[2] and ws[0]
+    assert need(2) == ws[2] + ws[0]
+    monkeypatch.delenv("FCAMD_BATCH_LANES")
+    assert lib.fcamd_batch_workspace_bytes(n, dims, kdims) == need(4)
+
+
 def test_gpu_mem_needed_mb(fc):
     assert fc.gpu_mem_needed_mb([256, 512, 512]) == (256 * 512 * 272 * 8) >> 20
     assert fc.lib().gpu_mem_needed_mb(None, 3) == -1
