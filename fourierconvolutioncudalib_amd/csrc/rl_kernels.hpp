@@ -12,6 +12,8 @@
 //                IFFTz (correlations add in the spectral domain)
 //   sk_xupdate   x-C2R of the summed correlation with the epilogue
 //                psi <- psi * acc / V
+// (the x kernels also in flat forms, staging the workgroup's rows through
+// LDS with coalesced HBM access; used when the real rows are dense)
 #pragma once
 
 #include "static_kernels_impl.hpp"
@@ -205,6 +207,133 @@ sk_xupdate(RLXArgs a) {
   });
 }
 
+// Flat forms of the two x kernels: the workgroup's spectrum rows and real
+// rows (phi, psi) move between HBM and LDS staging as coalesced runs, as in
+// the flat x passes (xflat_* in static_kernels_impl.hpp).  Real rows must be
+// dense (rrow == L).
+template <int L, int PAIRS>
+__global__ void __launch_bounds__(PAIRS * threads_per_transform<L>())
+sk_xratio_flat(RLXArgs a) {
+  using P = PlanOf<L, false>;
+  constexpr int T = P::T, LP = XRow<L>::LP, SL = XStage<L>::SL, NB = XStage<L>::NB;
+  constexpr int NT = PAIRS * T, MAXR = 2 * PAIRS;
+  __shared__ float2 lds[xflat_lds_floats<L>(PAIRS) / 2];
+  const long groups = ((a.nrows + 1) / 2 + PAIRS - 1) / PAIRS;
+  const int v = (int)(blockIdx.x / groups);
+  const long r0 = 2 * (blockIdx.x - (long)v * groups) * PAIRS;
+  const int nr = a.nrows - r0 < MAXR ? (int)(a.nrows - r0) : MAXR;
+  float2* spec = a.spec[v];
+  xflat_load_bins<L, NT, MAXR>(spec + r0 * a.srow, a.srow, lds, nr);
+  __syncthreads();
+  const int t = threadIdx.x % T, c = threadIdx.x / T;
+  const bool oka = 2 * c < nr, okb = 2 * c + 1 < nr;
+  const float2* sa = lds + 2 * c * NB;
+  const float2* sb = sa + NB;
+  float2 z[P::MAXE()];
+  // C2R on the reversed plan: its outputs sit where the forward plan reads
+  sfft_for_inputs<L, true>(t, [&](int slot, int pos) {
+    const bool mirror = pos >= NB;
+    const int kb = mirror ? L - pos : pos;
+    float2 A = oka ? sa[kb] : make_float2(0.f, 0.f);
+    float2 B = okb ? sb[kb] : make_float2(0.f, 0.f);
+    if (kb == 0 || 2 * kb == L) {
+      A.y = 0.f;
+      B.y = 0.f;
+    }
+    if (mirror) {
+      A.y = -A.y;
+      B.y = -B.y;
+    }
+    z[slot] = make_float2(A.x - B.y, A.y + B.x);
+  });
+  __syncthreads();
+  float2* row = lds + c * LP;
+  auto addr = [&](int pos) { return XRow<L>::at(pos); };
+  sfft_run<L, true, +1>(z, t, row, addr, a.fft.stw);
+  // phi rows through LDS; ratio at the transform's output positions
+  float* st = reinterpret_cast<float*>(lds);
+  xflat_load_rows<L, NT, MAXR>(a.phi[v] + r0 * L, st, nr);
+  __syncthreads();
+  const float eps = a.eps;
+  sfft_for_outputs<L, true>(t, [&](int slot, int pos) {
+    const float ya = oka ? st[2 * c * SL + pos] / fmaxf(z[slot].x, eps) : 0.f;
+    const float yb = okb ? st[(2 * c + 1) * SL + pos] / fmaxf(z[slot].y, eps) : 0.f;
+    z[slot] = make_float2(ya, yb);
+  });
+  __syncthreads();
+  sfft_run<L, false, -1>(z, t, row, addr, a.fft.stw);
+  sfft_for_outputs<L, false>(t, [&](int slot, int pos) { row[addr(pos)] = z[slot]; });
+  __syncthreads();
+  // Hermitian split back into the view's spectrum rows, bin-contiguous
+  constexpr int IT = (MAXR * NB + NT - 1) / NT;
+  const int n = nr * NB;
+#pragma unroll
+  for (int j = 0; j < IT; ++j) {
+    const int i = threadIdx.x + j * NT;
+    if (i < n) {
+      const int r = i / NB, kb = i - r * NB;
+      const float2* pr = lds + (r >> 1) * LP;
+      const float2 Z = pr[addr(kb)];
+      const float2 Zm = pr[addr(kb == 0 ? 0 : L - kb)];
+      spec[(r0 + r) * a.srow + kb] = (r & 1) ? make_float2(0.5f * (Z.y + Zm.y), 0.5f * (Zm.x - Z.x))
+                                            : make_float2(0.5f * (Z.x + Zm.x), 0.5f * (Z.y - Zm.y));
+    }
+  }
+}
+
+template <int L, int PAIRS>
+__global__ void __launch_bounds__(PAIRS * threads_per_transform<L>())
+sk_xupdate_flat(RLXArgs a) {
+  using P = PlanOf<L, false>;
+  constexpr int T = P::T, LP = XRow<L>::LP, SL = XStage<L>::SL, NB = XStage<L>::NB;
+  constexpr int NT = PAIRS * T, MAXR = 2 * PAIRS;
+  __shared__ float2 lds[xflat_lds_floats<L>(PAIRS) / 2];
+  const long r0 = 2 * (long)blockIdx.x * PAIRS;
+  const int nr = a.nrows - r0 < MAXR ? (int)(a.nrows - r0) : MAXR;
+  xflat_load_bins<L, NT, MAXR>(a.spec[0] + r0 * a.srow, a.srow, lds, nr);
+  __syncthreads();
+  const int t = threadIdx.x % T, c = threadIdx.x / T;
+  const bool oka = 2 * c < nr, okb = 2 * c + 1 < nr;
+  const float2* sa = lds + 2 * c * NB;
+  const float2* sb = sa + NB;
+  float2 z[P::MAXE()];
+  sfft_for_inputs<L, false>(t, [&](int slot, int pos) {
+    const bool mirror = pos >= NB;
+    const int kb = mirror ? L - pos : pos;
+    float2 A = oka ? sa[kb] : make_float2(0.f, 0.f);
+    float2 B = okb ? sb[kb] : make_float2(0.f, 0.f);
+    if (kb == 0 || 2 * kb == L) {
+      A.y = 0.f;
+      B.y = 0.f;
+    }
+    if (mirror) {
+      A.y = -A.y;
+      B.y = -B.y;
+    }
+    z[slot] = make_float2(A.x - B.y, A.y + B.x);
+  });
+  __syncthreads();
+  float2* row = lds + c * LP;
+  auto addr = [&](int pos) { return XRow<L>::at(pos); };
+  sfft_run<L, false, +1>(z, t, row, addr, a.fft.stw);
+  // psi rows through LDS: psi <- psi * acc / V at the output positions
+  float* st = reinterpret_cast<float*>(lds);
+  float* psi = a.psi + r0 * L;
+  xflat_load_rows<L, NT, MAXR>(psi, st, nr);
+  __syncthreads();
+  const float iv = a.inv_views;
+  sfft_for_outputs<L, false>(t, [&](int slot, int pos) {
+    if (oka) st[2 * c * SL + pos] *= z[slot].x * iv;
+    if (okb) st[(2 * c + 1) * SL + pos] *= z[slot].y * iv;
+  });
+  __syncthreads();
+  xflat_store_rows<L, NT, MAXR>(psi, st, nr);
+}
+
+#ifndef FC_RL_XFLAT
+#define FC_RL_XFLAT 1
+#endif
+
 template <int L>
 void rl_launch_zfan(const RLZArgs& a, long ny, hipStream_t st) {
   constexpr int TX = fit_tile<L, 16>();
@@ -223,9 +352,17 @@ void rl_launch_zsum(const RLZArgs& a, long ny, hipStream_t st) {
                      dim3(TX * threads_per_transform<L>()), 0, st, b);
 }
 
+// the flat forms need dense real rows; the per-pair forms take any stride
 template <int L>
 void rl_launch_xratio(const RLXArgs& a, int nviews, hipStream_t st) {
-  constexpr int P = batch_pairs<L>();
+  if (FC_RL_XFLAT && a.rrow == L) {
+    constexpr int P = flat_pairs<L, 256>();
+    const long groups = ((a.nrows + 1) / 2 + P - 1) / P;
+    hipLaunchKernelGGL((sk_xratio_flat<L, P>), dim3((unsigned)(groups * nviews)),
+                       dim3(P * threads_per_transform<L>()), 0, st, a);
+    return;
+  }
+  constexpr int P = fit_pairs<L, pairs_for<L, 256>()>();
   const long groups = ((a.nrows + 1) / 2 + P - 1) / P;
   hipLaunchKernelGGL((sk_xratio<L, P>), dim3((unsigned)(groups * nviews)),
                      dim3(P * threads_per_transform<L>()), 0, st, a);
@@ -233,7 +370,14 @@ void rl_launch_xratio(const RLXArgs& a, int nviews, hipStream_t st) {
 
 template <int L>
 void rl_launch_xupdate(const RLXArgs& a, hipStream_t st) {
-  constexpr int P = batch_pairs<L>();
+  if (FC_RL_XFLAT && a.rrow == L) {
+    constexpr int P = flat_pairs<L, 256>();
+    const long groups = ((a.nrows + 1) / 2 + P - 1) / P;
+    hipLaunchKernelGGL((sk_xupdate_flat<L, P>), dim3((unsigned)groups),
+                       dim3(P * threads_per_transform<L>()), 0, st, a);
+    return;
+  }
+  constexpr int P = fit_pairs<L, pairs_for<L, 256>()>();
   const long groups = ((a.nrows + 1) / 2 + P - 1) / P;
   hipLaunchKernelGGL((sk_xupdate<L, P>), dim3((unsigned)groups), dim3(P * threads_per_transform<L>()),
                      0, st, a);
