@@ -47,10 +47,11 @@ def pass_bytes(nx, ny, nz, kx, ky, kz):
     }, 8 * N + 64 * Nc + 4 * kx * ky * kz
 
 
-def cpu_baseline(img, psf):
+def cpu_baseline(img, psf, max_planes=None, single=True):
     """Reference host convolution (oracle/_ref, else its C restatement) over
-    the whole workload: the output z-planes are split into one contiguous
-    slab per host thread."""
+    the workload's output z-planes (all of them, or the first `max_planes`),
+    split into one contiguous slab per host thread; with `single`, also one
+    thread on one output plane."""
     import numpy as np
     from oracle import oracle as O
     kind = "reference" if O.ref_available() else "port"
@@ -58,7 +59,8 @@ def cpu_baseline(img, psf):
     threads = max(1, min(16, os.cpu_count() or 1))
     kz, ky, kx = psf.shape
     hz, hy, hx = kz // 2, ky // 2, kx // 2
-    nz = img.shape[0]
+    nz_all, ny, nx = img.shape
+    nz = nz_all if max_planes is None else min(nz_all, max_planes)
     per = -(-nz // threads)
     padded = np.pad(img, ((hz, hz), (hy, hy), (hx, hx)))
     slabs = []
@@ -78,15 +80,68 @@ def cpu_baseline(img, psf):
     for t in ths:
         t.join()
     dt = time.perf_counter() - t0
-    voxels = nz * img.shape[1] * img.shape[2]
-    return {
-        "value": voxels / dt,
+    what = (f"the whole {nx}x{ny}x{nz_all} k{kx}x{ky}x{kz} workload (all {nz} output z-planes"
+            if nz == nz_all else
+            f"{nz} of the {nz_all} output z-planes of the {nx}x{ny}x{nz_all} k{kx}x{ky}x{kz} "
+            f"workload (extrapolated per voxel")
+    res = {
+        "value": nz * ny * nx / dt,
         "unit": "voxels/s",
         "cores": len(slabs),
         "kind": kind,
-        "sample": (f"the whole {img.shape[2]}x{img.shape[1]}x{nz} k{kx}x{ky}x{kz} workload "
-                   f"(all {nz} output z-planes; zero-padded direct convolution, "
-                   f"tests/test_algorithms.hpp:9-58), {per} planes per thread, {dt:.1f} s"),
+        "sample": (f"{what}; zero-padded direct convolution, tests/test_algorithms.hpp:9-58), "
+                   f"{per} planes per thread, {dt:.1f} s"),
+    }
+    if single:
+        t1 = time.perf_counter()
+        fn(np.ascontiguousarray(padded[:1 + 2 * hz]), psf, (hz, hy, hx))
+        dt1 = time.perf_counter() - t1
+        res["single_thread"] = {"value": ny * nx / dt1, "unit": "voxels/s",
+                                "sample": f"1 output z-plane on 1 thread, {dt1:.2f} s"}
+    return res
+
+
+def config_timing(fc, torch, cfg, steps, cpu_planes, cpu_single):
+    """A further workload timed like the metric: device-resident, autotuned
+    variants (restored afterwards), HIP events around `steps` calls; its
+    whole-convolution roofline fraction; the reference host convolution on
+    up to `cpu_planes` output planes."""
+    import ctypes
+    from fourierconvolutioncudalib_amd import synth
+    img_np, psf_np = synth.workload(cfg)
+    nz, ny, nx = img_np.shape
+    kz, ky, kx = psf_np.shape
+    img = torch.from_numpy(img_np).cuda()
+    psf = torch.from_numpy(psf_np).cuda()
+    out = torch.empty_like(img)
+    ws = torch.empty(fc.workspace_bytes(img.shape, psf.shape), dtype=torch.uint8, device="cuda")
+    stream = torch.cuda.current_stream()
+    lib = fc.lib()
+    # selections of the lengths the tuner may change (-1: no compile-time kernel)
+    keep = {(p, L): lib.fcamd_selected_static_variant(p, L)
+            for p, L in ((0, nx), (1, nx), (2, ny), (3, nz))}
+    fc.autotune(img.shape, psf.shape, reps=3, stream=stream)
+    variants = fc.selected_variants(img.shape)
+    for _ in range(3):
+        fc.convolve3d(img, psf, out=out, workspace=ws, stream=stream)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        fc.convolve3d(img, psf, out=out, workspace=ws, stream=stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    for (p, L), v in keep.items():
+        if v >= 0:
+            lib.fcamd_select_static_variant(p, L, v)
+    _, conv_bytes = pass_bytes(nx, ny, nz, kx, ky, kz)
+    N = nx * ny * nz
+    return {
+        "workload": cfg["name"], "ms_per_conv": ms, "voxels_per_s": N / (ms * 1e-3),
+        "conv_roofline_frac": conv_bytes / (ms * 1e-3) / HBM_PEAK, "variants": variants,
+        "cpu_baseline": cpu_baseline(img_np, psf_np, cpu_planes, cpu_single),
     }
 
 
@@ -226,8 +281,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
-    ap.add_argument("--only", choices=["batched", "rl"],
-                    help="run one side comparison alone and print its JSON (for profiling)")
+    ap.add_argument("--only", choices=["batched", "rl", "configs"],
+                    help="run one side measurement alone and print its JSON (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -236,8 +291,14 @@ def main():
     if args.only:
         import torch
         import fourierconvolutioncudalib_amd as fc
-        side = batched_comparison if args.only == "batched" else rl_comparison
-        print(json.dumps({args.only: side(fc, torch)}))
+        from fourierconvolutioncudalib_amd import synth
+        sides = {
+            "batched": lambda: batched_comparison(fc, torch),
+            "rl": lambda: rl_comparison(fc, torch),
+            "configs": lambda: [config_timing(fc, torch, synth.CFG1, 50, None, True),
+                                config_timing(fc, torch, synth.CFG2, 20, 16, False)],
+        }
+        print(json.dumps({args.only: sides[args.only]()}))
         return
 
     import numpy as np
@@ -375,6 +436,8 @@ def main():
         result["boundary_modes"] = boundary_comparison(fc, torch, img, psf)
         result["batched"] = batched_comparison(fc, torch)
         result["richardson_lucy"] = rl_comparison(fc, torch)
+        result["configs"] = [config_timing(fc, torch, synth.CFG1, 50, None, True),
+                             config_timing(fc, torch, synth.CFG2, 20, 16, False)]
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(img_np, psf_np)
     else:
