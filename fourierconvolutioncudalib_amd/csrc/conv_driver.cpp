@@ -237,6 +237,11 @@ void pass_timing_enable(bool on) {
   g_pt_on = on;
 }
 
+bool pass_timing_enabled() {
+  std::lock_guard<std::mutex> lk(g_pt_mu);
+  return g_pt_on;
+}
+
 void pass_timing_collect(double* ms, int* count) {
   std::vector<PassEvents> ev;
   {

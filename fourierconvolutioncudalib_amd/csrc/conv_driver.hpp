@@ -60,6 +60,7 @@ const char* pass_name(int p);
 
 // Optional HIP-event timing of every pass launch (off by default).
 void pass_timing_enable(bool on);
+bool pass_timing_enabled();
 // Waits for the recorded events and returns the summed milliseconds and the
 // launch count per pass since the last call (then resets).
 void pass_timing_collect(double* ms, int* count);

@@ -3,11 +3,14 @@
 // For one convolution shape it times every variant of every pass that has
 // more than one (x-R2C row pairs, x-C2R row pairs, y tile shape / pipelining
 // / radix order, z kernel-spectrum method) with the others fixed, keeps the
-// fastest, and moves to the next pass (coordinate descent over passes).  The
+// fastest, and moves to the next pass (coordinate descent over passes).  A
+// variant is timed by the per-pass events of the launches it runs.  The
 // choice is per (pass, transform length) and process-wide.  With
 // FCAMD_TUNE_CACHE set, choices are appended to that file as
-// "pass length variant-name milliseconds" lines and re-applied by
+// "pass length variant-name milliseconds" lines (milliseconds per
+// convolution in that pass's launches) and re-applied by
 // fcamd_autotune_load() (called on the first tuning request).
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -40,16 +43,35 @@ void apply_cached(const std::string& path) {
   }
 }
 
-float time_convolution(const ConvShape& s, float* out, const float* img, const float* ker, void* ws,
-                       hipStream_t st, int reps, hipEvent_t e0, hipEvent_t e1) {
+// Milliseconds per convolution spent in the launches that run `pass`'s
+// variant (x forward, x inverse, both y passes, or the fused z pass), from
+// the per-pass events over `reps` convolutions; the best of three rounds.
+// Timing only the affected passes keeps the other passes' run-to-run noise
+// (a few microseconds each) out of the comparison.
+float time_variant(const ConvShape& s, int pass, float* out, const float* img, const float* ker,
+                   void* ws, hipStream_t st, int reps) {
   for (int i = 0; i < 2; ++i) enqueue_convolution(s, out, img, ker, Op::kConvolve, ws, st);
-  FC_HIP_CHECK(hipEventRecord(e0, st));
-  for (int i = 0; i < reps; ++i) enqueue_convolution(s, out, img, ker, Op::kConvolve, ws, st);
-  FC_HIP_CHECK(hipEventRecord(e1, st));
-  FC_HIP_CHECK(hipEventSynchronize(e1));
-  float ms = 0.f;
-  FC_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-  return ms / (float)reps;
+  FC_HIP_CHECK(hipStreamSynchronize(st));
+  double ms[kNumPasses];
+  int cnt[kNumPasses];
+  pass_timing_collect(ms, cnt);  // drops events recorded before
+  float best = 1e30f;
+  for (int round = 0; round < 3; ++round) {
+    pass_timing_enable(true);
+    for (int i = 0; i < reps; ++i) enqueue_convolution(s, out, img, ker, Op::kConvolve, ws, st);
+    pass_timing_enable(false);
+    FC_HIP_CHECK(hipStreamSynchronize(st));
+    pass_timing_collect(ms, cnt);
+    double t = 0.0;
+    switch (pass) {
+      case kSxFwd: t = ms[kPassXForward]; break;
+      case kSxInv: t = ms[kPassXInverse]; break;
+      case kSCol: t = ms[kPassYForward] + ms[kPassYInverse]; break;
+      default: t = ms[kPassZFused]; break;
+    }
+    best = std::min(best, (float)(t / reps));
+  }
+  return best;
 }
 
 template <class F>
@@ -100,14 +122,15 @@ int fcamd_autotune(const int* imDim, const int* kernelDim, int reps, void* strea
     const size_t nws = workspace_layout(s).total();
     float *img = nullptr, *out = nullptr, *ker = nullptr;
     void* ws = nullptr;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
+    // per-pass events are the tuner's clock; a caller's own pass timing is
+    // suspended meanwhile (its pending events are consumed) and restored
+    const bool timing = pass_timing_enabled();
     auto cleanup = [&] {
+      pass_timing_enable(timing);
       if (img) (void)hipFree(img);
       if (out) (void)hipFree(out);
       if (ker) (void)hipFree(ker);
       if (ws) (void)hipFree(ws);
-      if (e0) (void)hipEventDestroy(e0);
-      if (e1) (void)hipEventDestroy(e1);
     };
     try {
       FC_HIP_CHECK(hipMalloc(&img, nimg));
@@ -118,8 +141,6 @@ int fcamd_autotune(const int* imDim, const int* kernelDim, int reps, void* strea
       // values finite (0x3c3c3c3c ~ 0.0115f)
       FC_HIP_CHECK(hipMemsetAsync(img, 0x3c, nimg, st));
       FC_HIP_CHECK(hipMemsetAsync(ker, 0x3c, nker, st));
-      FC_HIP_CHECK(hipEventCreate(&e0));
-      FC_HIP_CHECK(hipEventCreate(&e1));
       const int axis_len[kSNumPasses] = {s.nx, s.nx, s.ny, s.nz};
       const char* cache = std::getenv("FCAMD_TUNE_CACHE");
       for (int pass = 0; pass < kSNumPasses; ++pass) {
@@ -129,7 +150,7 @@ int fcamd_autotune(const int* imDim, const int* kernelDim, int reps, void* strea
         float best_ms = 1e30f;
         for (int v = 0; v < e->count(pass); ++v) {
           e->selected[pass] = v;
-          const float ms = time_convolution(s, out, img, ker, ws, st, reps, e0, e1);
+          const float ms = time_variant(s, pass, out, img, ker, ws, st, reps);
           if (ms < best_ms) {
             best_ms = ms;
             best = v;
