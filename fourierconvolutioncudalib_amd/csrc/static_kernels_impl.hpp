@@ -251,54 +251,6 @@ __device__ __forceinline__ void xinv_lds_body(const XPassArgs& a, long blk, floa
   });
 }
 
-// x-C2R of a virtually padded volume writing only the cropped interior
-template <int L, int PAIRS>
-__global__ void __launch_bounds__(PAIRS * threads_per_transform<L>())
-sk_xinv_crop(XPassArgs a) {
-  __shared__ float2 lds[PAIRS * XRow<L>::LP];
-  using P = PlanOf<L, false>;
-  constexpr int T = P::T, LP = XRow<L>::LP;
-  const int t = threadIdx.x % T, c = threadIdx.x / T;
-  const long ra = 2 * ((long)blockIdx.x * PAIRS + c), rb = ra + 1;
-  const VPad& vp = a.vp;
-  long oa = 0, ob = 0;
-  const bool oka = ra < a.nrows && vpad_out_row(vp, ra, &oa);
-  const bool okb = rb < a.nrows && vpad_out_row(vp, rb, &ob);
-  const bool ina = ra < a.nrows, inb = rb < a.nrows;
-  const float2* pa = a.csrc + ra * a.src_row;
-  const float2* pb = a.csrc + rb * a.src_row;
-  const int nb = a.nbins;
-  float2 v[P::MAXE()];
-  sfft_for_inputs<L, false>(t, [&](int slot, int pos) {
-    const bool mirror = pos >= nb;
-    const int kb = mirror ? L - pos : pos;
-    float2 A = make_float2(0.f, 0.f), B = make_float2(0.f, 0.f);
-    if (ina && oka) A = pa[kb];
-    if (inb && okb) B = pb[kb];
-    if (kb == 0 || 2 * kb == L) {
-      A.y = 0.f;
-      B.y = 0.f;
-    }
-    if (mirror) {
-      A.y = -A.y;
-      B.y = -B.y;
-    }
-    v[slot] = make_float2(A.x - B.y, A.y + B.x);
-  });
-  float2* row = lds + c * LP;
-  auto addr = [&](int pos) { return XRow<L>::at(pos); };
-  sfft_run<L, false, +1>(v, t, row, addr, a.fft.stw);
-  float* da = a.rdst + oa * vp.m[0];
-  float* db = a.rdst + ob * vp.m[0];
-  const int x0 = vp.lo[0], x1 = vp.lo[0] + vp.m[0];
-  sfft_for_outputs<L, false>(t, [&](int slot, int pos) {
-    if (pos >= x0 && pos < x1) {
-      if (oka) da[pos - x0] = v[slot].x;
-      if (okb) db[pos - x0] = v[slot].y;
-    }
-  });
-}
-
 template <int L, int PAIRS>
 __global__ void __launch_bounds__(PAIRS * threads_per_transform<L>())
 sk_xinv_lds(XPassArgs a) {
@@ -567,6 +519,64 @@ sk_xinv_flat(XPassArgs a) {
   static_assert(xflat_lds_floats<L>(PAIRS) * 4 <= 65536, "flat x pass LDS");
   __shared__ float2 lds[xflat_lds_floats<L>(PAIRS) / 2];
   xinv_flat_body<L, PAIRS>(a, blockIdx.x, lds);
+}
+
+// Flat form of the cropped x-C2R of a virtually padded volume (VPad in
+// kernels.hpp): the workgroup's spectrum rows are staged as in the flat x
+// passes, and each kept row's interior [lo, lo + m) is written with
+// coalesced stores.  (A flat forward over the boundary maps, staging whole
+// source rows, measured slower than the per-pair body: 258 vs 171 us at
+// 600 x 600 x 288.)
+template <int L, int PAIRS>
+__global__ void __launch_bounds__(PAIRS * threads_per_transform<L>())
+sk_xinv_crop_flat(XPassArgs a) {
+  using P = PlanOf<L, false>;
+  constexpr int T = P::T, LP = XRow<L>::LP, SL = XStage<L>::SL, NB = XStage<L>::NB;
+  constexpr int NT = PAIRS * T, MAXR = 2 * PAIRS;
+  static_assert(xflat_lds_floats<L>(PAIRS) * 4 <= 65536, "flat x pass LDS");
+  __shared__ float2 lds[xflat_lds_floats<L>(PAIRS) / 2];
+  const long r0 = 2 * (long)blockIdx.x * PAIRS;
+  const int nr = a.nrows - r0 < MAXR ? (int)(a.nrows - r0) : MAXR;
+  xflat_load_bins<L, NT, MAXR>(a.csrc + r0 * a.src_row, a.src_row, lds, nr);
+  __syncthreads();
+  const int t = threadIdx.x % T, c = threadIdx.x / T;
+  const bool oka = 2 * c < nr, okb = 2 * c + 1 < nr;
+  const float2* sa = lds + 2 * c * NB;
+  const float2* sb = sa + NB;
+  float2 v[P::MAXE()];
+  sfft_for_inputs<L, false>(t, [&](int slot, int pos) {
+    const bool mirror = pos >= NB;
+    const int kb = mirror ? L - pos : pos;
+    float2 A = oka ? sa[kb] : make_float2(0.f, 0.f);
+    float2 B = okb ? sb[kb] : make_float2(0.f, 0.f);
+    if (kb == 0 || 2 * kb == L) {
+      A.y = 0.f;
+      B.y = 0.f;
+    }
+    if (mirror) {
+      A.y = -A.y;
+      B.y = -B.y;
+    }
+    v[slot] = make_float2(A.x - B.y, A.y + B.x);
+  });
+  __syncthreads();
+  float2* row = lds + c * LP;
+  auto addr = [&](int pos) { return XRow<L>::at(pos); };
+  sfft_run<L, false, +1>(v, t, row, addr, a.fft.stw);
+  float* st = reinterpret_cast<float*>(lds);
+  sfft_for_outputs<L, false>(t, [&](int slot, int pos) {
+    st[2 * c * SL + pos] = v[slot].x;
+    st[(2 * c + 1) * SL + pos] = v[slot].y;
+  });
+  __syncthreads();
+  const VPad& vp = a.vp;
+  const int m0 = vp.m[0], x0 = vp.lo[0];
+  for (int r = 0; r < nr; ++r) {
+    long orow = 0;
+    if (!vpad_out_row(vp, r0 + r, &orow)) continue;  // uniform across the workgroup
+    float* g = a.rdst + orow * m0;
+    for (int x = threadIdx.x; x < m0; x += NT) g[x] = st[r * SL + x0 + x];
+  }
 }
 
 // batched x passes: workgroup i serves tasks[i] = {argument record, row-pair
@@ -1014,10 +1024,10 @@ bool sk_launch_xfwd_vpad(const XPass2& a, hipStream_t st) {
 }
 
 template <int L, int TGT>
-bool sk_launch_xinv_crop(const XPassArgs& a, hipStream_t st) {
-  constexpr int P = fit_pairs<L, pairs_for<L, TGT>()>();
+bool sk_launch_xinv_crop_flat(const XPassArgs& a, hipStream_t st) {
+  constexpr int P = flat_pairs<L, TGT>();
   const long blocks = ((a.nrows + 1) / 2 + P - 1) / P;
-  hipLaunchKernelGGL((sk_xinv_crop<L, P>), dim3((unsigned)blocks),
+  hipLaunchKernelGGL((sk_xinv_crop_flat<L, P>), dim3((unsigned)blocks),
                      dim3(P * threads_per_transform<L>()), 0, st, a);
   return true;
 }
@@ -1201,7 +1211,7 @@ StaticEntry make_static_entry() {
   e.selected[kSxInv] = index_of(e.xinv, "w256-flat");
   if (L < 128 || L % 4 != 0) e.selected[kSxFwd] = index_of(e.xfwd, "w256-flat");
   e.xfwd_vpad = &sk_launch_xfwd_vpad<L, 64>;
-  e.xinv_crop = &sk_launch_xinv_crop<L, 64>;
+  e.xinv_crop = &sk_launch_xinv_crop_flat<L, 256>;  // 171 vs 193 us at 600 x 600 x 288
   e.batch.pairs = batch_pairs<L>();
   e.batch.tx = fit_tile<L, 16>();
   e.batch.xfwd = &sb_launch_xfwd<L>;
