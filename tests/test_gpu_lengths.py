@@ -117,6 +117,9 @@ def test_flat_x_variants_many_workgroups(fc, oracle, L):
     ref = oracle.numpy_fft_conv(img, k)
     tk = torch.from_numpy(k).cuda()
     n = img.size
+    # one row and three rows (a single, ragged workgroup)
+    few = [(synth.uniform(s, 33), synth.random_psf((1, min(3, s[1]), 5), 6)) for s in
+           ((1, 1, L), (1, 3, L))]
     prev = {p: lib.fcamd_selected_static_variant(p, L) for p in (0, 1)}
     try:
         for name in ("w256-flat", "w128-flat"):
@@ -126,6 +129,8 @@ def test_flat_x_variants_many_workgroups(fc, oracle, L):
                 assert lib.fcamd_select_static_variant(p, L, names.index(name)) == 0
             out = fc.convolve(img, k)
             assert oracle.rel_l2(out, ref) < TOL, (name, "aligned")
+            for a, kk in few:
+                assert oracle.rel_l2(fc.convolve(a, kk), oracle.numpy_fft_conv(a, kk)) < TOL, a.shape
             buf = torch.zeros(n + 1, device="cuda")
             ti = buf[1:].view(shape)
             ti.copy_(torch.from_numpy(img).cuda())
