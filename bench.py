@@ -195,6 +195,45 @@ def batched_comparison(fc, torch, reps=5):
     }
 
 
+def adjoint_timing(fc, torch, reps=10):
+    """Differentiation workload, 256x256x128 with a 21^3 PSF: forward
+    convolution, image gradient (correlation with the PSF) and PSF gradient
+    (correlation of the output gradient with the image on the PSF support),
+    device-resident, HIP events per part."""
+    from fourierconvolutioncudalib_amd import synth
+    from fourierconvolutioncudalib_amd.autograd import grad_kernel, grad_kernel_workspace_bytes
+    shape, kshape = (128, 256, 256), (21, 21, 21)
+    img = torch.from_numpy(synth.uniform(shape, 1003)).cuda()
+    g = torch.from_numpy(synth.uniform(shape, 1013)).cuda()
+    psf = torch.from_numpy(synth.gaussian_psf(kshape, (2.0, 2.5, 3.0))).cuda()
+    out = torch.empty_like(img)
+    gi = torch.empty_like(img)
+    ws = torch.empty(fc.workspace_bytes(shape, kshape), dtype=torch.uint8, device="cuda")
+    wk = torch.empty(grad_kernel_workspace_bytes(shape, kshape), dtype=torch.uint8, device="cuda")
+    stream = torch.cuda.current_stream()
+    parts = {
+        "forward_ms": lambda: fc.convolve3d(img, psf, out=out, workspace=ws, stream=stream),
+        "grad_image_ms": lambda: fc.convolve3d(g, psf, op=fc.CORRELATE, out=gi, workspace=ws,
+                                               stream=stream),
+        "grad_kernel_ms": lambda: grad_kernel(g, img, kshape, workspace=wk, stream=stream),
+    }
+    res = {"workload": "adjoint_256x256x128_k21"}
+    for name, fn in parts.items():
+        for _ in range(3):
+            fn()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        res[name] = e0.elapsed_time(e1) / reps
+    res["step_ms"] = res["forward_ms"] + res["grad_image_ms"] + res["grad_kernel_ms"]
+    return res
+
+
 def rl_comparison(fc, torch):
     """Multiview Richardson-Lucy, 4 views of 384x384x192 with 25x25x51 PSFs,
     10 iterations: fused pass sequence vs per-view convolutions."""
@@ -281,7 +320,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
-    ap.add_argument("--only", choices=["batched", "rl", "configs"],
+    ap.add_argument("--only", choices=["batched", "rl", "adjoint", "configs"],
                     help="run one side measurement alone and print its JSON (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -295,6 +334,7 @@ def main():
         sides = {
             "batched": lambda: batched_comparison(fc, torch),
             "rl": lambda: rl_comparison(fc, torch),
+            "adjoint": lambda: adjoint_timing(fc, torch),
             "configs": lambda: [config_timing(fc, torch, synth.CFG1, 50, None, True),
                                 config_timing(fc, torch, synth.CFG2, 20, 16, False)],
         }
@@ -436,6 +476,7 @@ def main():
         result["boundary_modes"] = boundary_comparison(fc, torch, img, psf)
         result["batched"] = batched_comparison(fc, torch)
         result["richardson_lucy"] = rl_comparison(fc, torch)
+        result["adjoint"] = adjoint_timing(fc, torch)
         result["configs"] = [config_timing(fc, torch, synth.CFG1, 50, None, True),
                              config_timing(fc, torch, synth.CFG2, 20, 16, False)]
     if not args.no_cpu_baseline:
