@@ -113,8 +113,9 @@ FCAMD_API void fcamd_conv_plan_destroy(void* plan);
  * convolutions, block b with dims imDims[3b..3b+2] ({x,y,z}), kernel dims
  * kernelDims[3b..3b+2], device buffers d_images[b], d_kernels[b] and
  * d_outs[b] (may alias the image).  fcamd_batch_execute enqueues the
- * sequence on `stream`, fcamd_batch_graph_launch captures it into a hipGraph
- * on the first call and replays that graph afterwards.  The buffers are
+ * sequence on `stream`, fcamd_batch_graph_launch captures it into hipGraphs
+ * on the first call (one per lane, or one for the length groups) and
+ * replays them afterwards, forked from and joined back into `stream`.  The buffers are
  * bound at creation.  Two schedules, chosen at creation by the environment:
  *   FCAMD_BATCH_LANES=N (unset = 4): the blocks are dealt to N streams
  *     ("lanes"), largest estimated cost first to the least-loaded lane;
