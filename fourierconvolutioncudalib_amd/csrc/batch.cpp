@@ -579,33 +579,51 @@ void BatchPlan::enqueue(hipStream_t st) {
 void BatchPlan::graph_launch(hipStream_t st) {
   if (!lanes_.empty()) {
     const size_t nl = lanes_.size();
-    if (lane_execs_.empty()) {
-      // each lane's pass sequence captured on a plan-owned stream
+    if (lane_execs_.size() != nl) {
+      // each lane's pass sequence captured on a plan-owned stream; the
+      // graphs are kept only once every lane has been captured and
+      // instantiated, so a failure leaves no partial set behind
       const Op op = conj_ ? Op::kCorrelate : Op::kConvolve;
-      for (size_t i = 0; i < nl; ++i) {
+      while (lane_caps_.size() < nl) {
         hipStream_t cs = nullptr;
         FC_HIP_CHECK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
         lane_caps_.push_back(cs);
-        FC_HIP_CHECK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-        try {
-          for (int b : lanes_[i]) {
-            const Block& bl = blocks_[b];
-            enqueue_convolution(bl.s, bl.out, bl.img, bl.ker, op,
-                                static_cast<char*>(ws_) + lane_off_[i], cs);
-          }
-        } catch (...) {
-          hipGraph_t g = nullptr;
-          (void)hipStreamEndCapture(cs, &g);
-          if (g) (void)hipGraphDestroy(g);
-          throw;
-        }
-        hipGraph_t g = nullptr;
-        FC_HIP_CHECK(hipStreamEndCapture(cs, &g));
-        lane_graphs_.push_back(g);
-        hipGraphExec_t x = nullptr;
-        FC_HIP_CHECK(hipGraphInstantiate(&x, g, nullptr, nullptr, 0));
-        lane_execs_.push_back(x);
       }
+      std::vector<hipGraph_t> graphs;
+      std::vector<hipGraphExec_t> execs;
+      auto drop = [&] {
+        for (auto x : execs) (void)hipGraphExecDestroy(x);
+        for (auto g : graphs) (void)hipGraphDestroy(g);
+      };
+      try {
+        for (size_t i = 0; i < nl; ++i) {
+          hipStream_t cs = lane_caps_[i];
+          FC_HIP_CHECK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+          try {
+            for (int b : lanes_[i]) {
+              const Block& bl = blocks_[b];
+              enqueue_convolution(bl.s, bl.out, bl.img, bl.ker, op,
+                                  static_cast<char*>(ws_) + lane_off_[i], cs);
+            }
+          } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(cs, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+          }
+          hipGraph_t g = nullptr;
+          FC_HIP_CHECK(hipStreamEndCapture(cs, &g));
+          graphs.push_back(g);
+          hipGraphExec_t x = nullptr;
+          FC_HIP_CHECK(hipGraphInstantiate(&x, g, nullptr, nullptr, 0));
+          execs.push_back(x);
+        }
+      } catch (...) {
+        drop();
+        throw;
+      }
+      lane_graphs_ = std::move(graphs);
+      lane_execs_ = std::move(execs);
     }
     // lane 0 on `st`, the others on side streams forked from and joined
     // back into `st`
@@ -634,8 +652,16 @@ void BatchPlan::graph_launch(hipStream_t st) {
       if (g) (void)hipGraphDestroy(g);
       throw;
     }
-    FC_HIP_CHECK(hipStreamEndCapture(cap_, &graph_));
-    FC_HIP_CHECK(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0));
+    hipGraph_t g = nullptr;
+    FC_HIP_CHECK(hipStreamEndCapture(cap_, &g));
+    hipGraphExec_t x = nullptr;
+    const hipError_t rc = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (rc != hipSuccess) {
+      (void)hipGraphDestroy(g);
+      FC_HIP_CHECK(rc);
+    }
+    graph_ = g;
+    exec_ = x;
   }
   FC_HIP_CHECK(hipGraphLaunch(exec_, st));
 }
