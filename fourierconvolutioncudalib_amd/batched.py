@@ -5,8 +5,9 @@ deals the blocks to a few streams ("lanes", balanced by estimated cost),
 each lane running its blocks' pass sequences in its own workspace slot;
 FCAMD_BATCH_LANES=0 selects the length-group schedule instead (one
 compile-time kernel launch per transform length and pass, length groups on
-forked streams).  Either sequence can be captured in a hipGraph that is
-replayed on later calls.  Buffers are bound when the plan is created.
+forked streams).  With graph replay, the first call captures the plan
+(one graph per lane, or one graph for the length groups) and every call
+launches the captured graphs.  Buffers are bound when the plan is created.
 """
 from __future__ import annotations
 
@@ -47,7 +48,8 @@ class BatchedConvolution:
 
     def run(self, stream=None, graph=True):
         """Enqueue all blocks on ``stream`` (default: torch's current stream);
-        ``graph`` replays the captured hipGraph."""
+        ``graph`` launches the captured hipGraphs (captured on the first
+        such call)."""
         if self._plan is None:
             raise RuntimeError("plan closed")
         with torch.cuda.device(self.device):
