@@ -128,7 +128,9 @@ FCAMD_API void fcamd_conv_plan_destroy(void* plan);
  *     workspace region.
  * d_workspace may be NULL (the plan allocates it) or hold
  * fcamd_batch_workspace_bytes() bytes (computed under the same settings).
- * create returns NULL on error. */
+ * Every run of one plan uses that plan's workspace, side streams and events,
+ * so runs of one plan must be ordered, e.g. all on one stream.  Runs on
+ * different streams that overlap race.  create returns NULL on error. */
 FCAMD_API long long fcamd_batch_workspace_bytes(int nblocks, const int* imDims,
                                                 const int* kernelDims);
 FCAMD_API void* fcamd_batch_create(int nblocks, const int* imDims, const int* kernelDims,
