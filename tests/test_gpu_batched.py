@@ -166,3 +166,22 @@ def test_batched_lanes(fc, oracle, monkeypatch, lanes):
         for g, p in zip(got, plain):
             assert oracle.rel_l2(g.cpu().numpy(), p) < 1e-6, rep
     plan.close()
+
+
+def test_batched_in_place_mixed_runs(fc, oracle, schedule):
+    """Outputs aliasing the images, with graph and eager runs of one plan
+    alternating: every run equals the plain call on the images it read."""
+    import torch
+    from fourierconvolutioncudalib_amd.batched import BatchedConvolution
+    imgs, ks = _blocks(700, SHAPES[:6])
+    ti = [torch.from_numpy(a).cuda() for a in imgs]
+    tk = [torch.from_numpy(k).cuda() for k in ks]
+    plan = BatchedConvolution(ti, tk, outs=ti)
+    for rep, graph in enumerate((True, False, True, True, False)):
+        want = [fc.convolve3d(a, k) for a, k in zip(ti, tk)]
+        torch.cuda.synchronize()
+        plan.run(graph=graph)
+        torch.cuda.synchronize()
+        for g, w in zip(ti, want):
+            assert oracle.rel_l2(g.cpu().numpy(), w.cpu().numpy()) < 1e-6, (rep, graph)
+    plan.close()
