@@ -383,6 +383,20 @@ def main():
     wall = time.perf_counter() - t0
     ms_step = e0.elapsed_time(e1) / args.steps
 
+    # per-call spread: at least 20 further calls, each between its own pair of
+    # events; the median is reported beside the mean of the timed region
+    n_med = max(20, args.steps)
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(n_med)]
+    for a, b in ev:
+        a.record(stream)
+        step()
+        b.record(stream)
+    torch.cuda.synchronize()
+    per_call = sorted(a.elapsed_time(b) for a, b in ev)
+    ms_median = per_call[n_med // 2] if n_med % 2 else 0.5 * (per_call[n_med // 2 - 1] +
+                                                                per_call[n_med // 2])
+
     # timed region 2: the same steps with HIP events around every pass
     lib.fcamd_pass_timing(1)
     torch.cuda.synchronize()
@@ -443,6 +457,8 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": ms_step,
+        "ms_per_call_median": ms_median,
+        "median_calls": n_med,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
