@@ -158,6 +158,8 @@ def batched_comparison(fc, torch, reps=5):
     ws = torch.empty(max(fc.workspace_bytes(s, k) for s, k, _ in spec), dtype=torch.uint8,
                      device="cuda")
     voxels = sum(int(t.numel()) for t in imgs)
+    # algorithmic bytes of the whole batch: the sum over its blocks
+    bytes_alg = sum(pass_bytes(s[2], s[1], s[0], k[2], k[1], k[0])[1] for s, k, _ in spec)
     stream = torch.cuda.current_stream()
 
     def plain():
@@ -190,6 +192,9 @@ def batched_comparison(fc, torch, reps=5):
         "plain_loop_ms": t_plain, "batched_eager_ms": t_eager, "batched_graph_ms": t_graph,
         "plain_loop_wall_ms": w_plain, "batched_graph_wall_ms": w_graph,
         "batched_voxels_per_s": voxels / (t_graph * 1e-3),
+        "bytes_alg": bytes_alg,
+        "conv_roofline_frac_graph": bytes_alg / (t_graph * 1e-3) / HBM_PEAK,
+        "conv_roofline_frac_plain": bytes_alg / (t_plain * 1e-3) / HBM_PEAK,
         "speedup_graph_vs_plain": t_plain / t_graph,
         "speedup_eager_vs_plain": t_plain / t_eager,
     }
