@@ -236,6 +236,12 @@ def adjoint_timing(fc, torch, reps=10):
         torch.cuda.synchronize()
         res[name] = e0.elapsed_time(e1) / reps
     res["step_ms"] = res["forward_ms"] + res["grad_image_ms"] + res["grad_kernel_ms"]
+    # the forward and the image gradient each move one convolution's
+    # algorithmic bytes
+    _, cb = pass_bytes(shape[2], shape[1], shape[0], kshape[2], kshape[1], kshape[0])
+    res["bytes_alg_per_conv"] = cb
+    res["forward_roofline_frac"] = cb / (res["forward_ms"] * 1e-3) / HBM_PEAK
+    res["grad_image_roofline_frac"] = cb / (res["grad_image_ms"] * 1e-3) / HBM_PEAK
     return res
 
 
