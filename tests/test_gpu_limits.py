@@ -1,0 +1,56 @@
+"""Transform lengths at the documented single-tile limits of the
+runtime-length kernels (x <= 10239, y <= 10240, z <= 6826: one row, column or
+z column of the transform held in 160 KiB of LDS), against the FP64 oracle,
+and one past each limit rejected with the unsupported-size status before any
+device work, leaving the image untouched.
+
+The exact limits 10239 = 3 * 3413 and 6826 = 2 * 3413 run a direct 3413-point
+DFT stage; 10206 = 2 * 3^6 * 7 and 10240 = 2^11 * 5 run butterflies only.
+Tolerance: rel-L2 <= 1e-5 against the FP64 oracle, as for every other shape.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+UNSUPPORTED_SIZE = 2
+
+
+def _run(fc, stack, kernel):
+    im = np.ascontiguousarray(stack, dtype=np.float32).copy()
+    k = np.ascontiguousarray(kernel, dtype=np.float32)
+    fc.convolution3DfftCUDAInPlace(im.reshape(-1), list(im.shape[::-1]), k.reshape(-1),
+                                   list(k.shape[::-1]), 0)
+    return im
+
+
+# (image C-order shape, kernel C-order shape)
+CASES = [((2, 3, 10239), (1, 3, 7)),    # x at its limit (prime stage 3413)
+         ((2, 3, 10206), (1, 3, 9)),    # x near its limit, butterflies only
+         ((2, 10240, 3), (1, 9, 3)),    # y at its limit
+         ((6826, 2, 3), (5, 1, 3)),     # z at its limit (prime stage 3413)
+         ((6720, 2, 2), (7, 1, 1))]     # z near its limit, butterflies only
+
+
+@pytest.mark.parametrize("shape,kshape", CASES, ids=lambda v: "x".join(map(str, v)))
+def test_maximum_extents(fc, oracle, shape, kshape):
+    from fourierconvolutioncudalib_amd import synth
+    print("case", shape, kshape, flush=True)
+    stack = synth.uniform(shape, 4100 + sum(shape))
+    kernel = synth.random_psf(kshape, 4200 + sum(kshape))
+    out = _run(fc, stack, kernel)
+    ref = oracle.fft_conv(stack, kernel)
+    assert oracle.rel_l2(out, ref) < TOL
+
+
+@pytest.mark.parametrize("shape", [(2, 3, 10240), (2, 10241, 3), (6827, 2, 3)],
+                         ids=lambda v: "x".join(map(str, v)))
+def test_past_limits_rejected(fc, shape):
+    im = np.ones(shape, np.float32)
+    k = np.ones((1, 1, 1), np.float32)
+    with pytest.raises(fc.FCError) as err:
+        fc.convolution3DfftCUDAInPlace(im.reshape(-1), list(shape[::-1]), k.reshape(-1),
+                                       [1, 1, 1], 0)
+    assert err.value.code == UNSUPPORTED_SIZE
+    assert np.all(im == 1)
