@@ -2,7 +2,8 @@
 runtime-length kernels (x <= 10239, y <= 10240, z <= 6826: one row, column or
 z column of the transform held in 160 KiB of LDS), against the FP64 oracle,
 and one past each limit rejected with the unsupported-size status before any
-device work, leaving the image untouched.
+device work, leaving the image untouched; zero extents are rejected as
+invalid arguments.
 
 The exact limits 10239 = 3 * 3413 and 6826 = 2 * 3413 run a direct 3413-point
 DFT stage; 10206 = 2 * 3^6 * 7 and 10240 = 2^11 * 5 run butterflies only.
@@ -14,6 +15,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 TOL = 1e-5
+INVALID_ARGUMENT = 1
 UNSUPPORTED_SIZE = 2
 
 
@@ -53,4 +55,18 @@ def test_past_limits_rejected(fc, shape):
         fc.convolution3DfftCUDAInPlace(im.reshape(-1), list(shape[::-1]), k.reshape(-1),
                                        [1, 1, 1], 0)
     assert err.value.code == UNSUPPORTED_SIZE
+    assert np.all(im == 1)
+
+
+@pytest.mark.parametrize("dims,kdims", [((0, 4, 4), (1, 1, 1)), ((4, 4, 4), (3, 0, 3))],
+                         ids=["empty-image", "empty-kernel"])
+def test_empty_extents_rejected(fc, dims, kdims):
+    """A zero extent is an invalid argument: no device work, and the image
+    (when it has voxels) stays untouched."""
+    im = np.ones(dims, np.float32)
+    k = np.ones(kdims, np.float32)
+    with pytest.raises(fc.FCError) as err:
+        fc.convolution3DfftCUDAInPlace(im.reshape(-1), list(dims[::-1]), k.reshape(-1),
+                                       list(kdims[::-1]), 0)
+    assert err.value.code == INVALID_ARGUMENT
     assert np.all(im == 1)
