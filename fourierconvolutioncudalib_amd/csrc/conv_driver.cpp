@@ -527,7 +527,19 @@ void enqueue_grad_kernel(const ConvShape& s, float* d_dkernel, const float* d_gr
   forward_xy(s, d_grad_out, G, nullptr, nullptr, nullptr, st);
   forward_xy(s, d_image, I, nullptr, nullptr, nullptr, st);
   z_pass(s, G, I, true, true, st);
-  inverse_yx(s, G, U, st);
+  // The gather reads u only on the z-planes (j - cz) mod nz, j < kz: the
+  // runs [0, kz - cz) and [nz - cz, nz), disjoint since kz <= nz.  The y and
+  // x inverses act within a plane, so each run is inverted as a volume of
+  // its own planes; the other planes of U are never written or read.
+  const long plane_bins = (long)padded_bins(s.nx) * s.ny;
+  const long plane_vox = (long)s.nx * s.ny;
+  const int runs[2][2] = {{0, s.kz - s.cz}, {s.nz - s.cz, s.nz}};
+  for (const auto& r : runs) {
+    if (r[1] <= r[0]) continue;
+    ConvShape sub = s;
+    sub.nz = r[1] - r[0];
+    inverse_yx(sub, G + r[0] * plane_bins, U + r[0] * plane_vox, st);
+  }
   launch_gather_kernel_support(s, U, d_dkernel, st);
   FC_HIP_CHECK(hipGetLastError());
 }
