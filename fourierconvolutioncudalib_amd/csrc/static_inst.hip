@@ -1,5 +1,5 @@
 // static_inst.hip -- instantiates the compile-time kernels for one group of
-// lengths (FC_GROUP, set by the Makefile) and registers them at load time.
+// lengths (FC_GROUP, set by the Makefile) and registers them on request.
 // The groups split the instantiations over several objects so they compile
 // in parallel.
 #include "rl_kernels.hpp"
@@ -28,10 +28,15 @@
 #endif
 
 namespace fcamd {
-namespace {
 
-struct Registrar {
-  Registrar() {
+#define FC_CAT2(a, b) a##b
+#define FC_CAT(a, b) FC_CAT2(a, b)
+
+// Registers this group's lengths.  Called by the registry on first lookup
+// (static_kernels.hip), so a program that links the static archive pulls
+// every group in through this reference; no load-time constructor is
+// relied on.
+void FC_CAT(static_register_group_, FC_GROUP)() {
 #define FC_REG(L)                                \
   {                                              \
     StaticEntry e = make_static_entry<L>();      \
@@ -41,11 +46,8 @@ struct Registrar {
     e.rl.xupdate = &rl_launch_xupdate<L>;        \
     static_register(L, e);                       \
   }
-    FC_LENGTHS(FC_REG)
+  FC_LENGTHS(FC_REG)
 #undef FC_REG
-  }
-};
-Registrar g_registrar;
+}
 
-}  // namespace
 }  // namespace fcamd

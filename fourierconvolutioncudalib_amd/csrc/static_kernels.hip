@@ -24,6 +24,31 @@ std::mutex& registry_mu() {
 // FCAMD_NO_STATIC=1 (or fcamd_use_static_kernels(0)) forces the
 // runtime-length kernels
 std::atomic<bool> g_disabled{false};
+std::once_flag g_registered;
+}  // namespace
+
+// one per instantiation group of static_inst.hip (FC_GROUP 0..5)
+void static_register_group_0();
+void static_register_group_1();
+void static_register_group_2();
+void static_register_group_3();
+void static_register_group_4();
+void static_register_group_5();
+
+namespace {
+// Fills the registry on first use.  Calling the groups by name (instead of
+// load-time constructors in their objects) keeps every group linked in when
+// a program links the static archive.
+void ensure_registered() {
+  std::call_once(g_registered, [] {
+    static_register_group_0();
+    static_register_group_1();
+    static_register_group_2();
+    static_register_group_3();
+    static_register_group_4();
+    static_register_group_5();
+  });
+}
 }  // namespace
 
 void static_register(int L, const StaticEntry& e) {
@@ -37,6 +62,7 @@ StaticEntry* static_lookup(int L) {
     return v != nullptr && v[0] == '1';
   }();
   if (env_disabled || g_disabled) return nullptr;
+  ensure_registered();
   std::lock_guard<std::mutex> lk(registry_mu());
   auto it = registry().find(L);
   return it == registry().end() ? nullptr : it->second.get();
@@ -178,6 +204,7 @@ int fcamd_has_static_kernel(int pass, int length) {
 void fcamd_use_static_kernels(int enable) { g_disabled = enable == 0; }
 
 int fcamd_static_lengths(int* lengths, int cap) {
+  ensure_registered();
   std::lock_guard<std::mutex> lk(registry_mu());
   int n = 0;
   for (auto& kv : registry()) {
