@@ -187,6 +187,52 @@ def selected_variants(image_shape):
     return out
 
 
+def _check_tensor(t, name, shape=None, device=None, contiguous=True):
+    """Raises before any library call unless ``t`` is a float32 CUDA tensor
+    (on ``device``, of exactly ``shape``, contiguous when asked): a pointer
+    of another dtype, a host pointer or a short buffer would otherwise reach
+    a device kernel."""
+    import torch
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{name} must be a torch tensor, got {type(t).__name__}")
+    if t.dtype != torch.float32:
+        raise TypeError(f"{name} must be float32, got {t.dtype}")
+    if not t.is_cuda:
+        raise TypeError(f"{name} must be a CUDA tensor (there is no CPU path)")
+    if device is not None and t.device != torch.device(device):
+        raise ValueError(f"{name} is on {t.device}, expected {device}")
+    if shape is not None and tuple(t.shape) != tuple(int(d) for d in shape):
+        raise ValueError(f"{name} has shape {tuple(t.shape)}, expected {tuple(shape)}")
+    if contiguous and not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+    return t
+
+
+def _workspace(nbytes, workspace, device, stream):
+    """(pointer, bytes, keep-alive tensor) of the workspace for one call:
+    the caller's, else a new one from torch's caching allocator (no
+    synchronisation, usable under graph capture).  A new workspace used on a
+    stream other than torch's current one is recorded on that stream so the
+    allocator does not hand it out again before the call's kernels ran."""
+    import torch
+    if workspace is not None:
+        if not isinstance(workspace, torch.Tensor) or not workspace.is_cuda:
+            raise TypeError("workspace must be a CUDA tensor")
+        if workspace.device != device or not workspace.is_contiguous():
+            raise ValueError("workspace must be a contiguous tensor on the image's device")
+        have = workspace.numel() * workspace.element_size()
+        if have < nbytes:
+            raise ValueError(f"workspace holds {have} bytes, {nbytes} needed")
+        return workspace.data_ptr(), have, workspace
+    ws = torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device=device)
+    if stream is not None:
+        if isinstance(stream, int):
+            stream = torch.cuda.ExternalStream(stream, device=device)
+        if stream.cuda_stream != torch.cuda.current_stream(device).cuda_stream:
+            ws.record_stream(stream)
+    return ws.data_ptr(), int(nbytes), ws
+
+
 def _stream_handle(stream):
     if stream is None:
         import torch
@@ -205,8 +251,9 @@ class ConvolutionPlan:
 
     def __init__(self, image_shape, kernel, op=CONVOLVE, stream=None, workspace=None):
         import torch
-        if kernel.dtype != torch.float32 or not kernel.is_cuda or kernel.dim() != 3:
-            raise TypeError("kernel must be a 3D float32 CUDA tensor")
+        _check_tensor(kernel, "kernel", contiguous=False)
+        if kernel.dim() != 3:
+            raise ValueError("kernel must be 3D")
         self.image_shape = tuple(int(d) for d in image_shape)
         self.kernel_shape = tuple(kernel.shape)
         self.device = kernel.device
@@ -214,7 +261,8 @@ class ConvolutionPlan:
         self._workspace = workspace
         wptr, wbytes = None, 0
         if workspace is not None:
-            wptr, wbytes = workspace.data_ptr(), workspace.numel() * workspace.element_size()
+            wptr, wbytes, _ = _workspace(workspace_bytes(self.image_shape, self.kernel_shape),
+                                         workspace, self.device, stream)
         with torch.cuda.device(self.device):
             self._plan = lib().fcamd_conv_plan_create(
                 ints(self.image_shape[::-1]), ints(self.kernel_shape[::-1]),
@@ -224,9 +272,8 @@ class ConvolutionPlan:
 
     def set_kernel(self, kernel, stream=None):
         """Replace the kernel (same dims); transformed on ``stream``."""
+        _check_tensor(kernel, "kernel", self.kernel_shape, self.device, contiguous=False)
         import torch
-        if tuple(kernel.shape) != self.kernel_shape or kernel.dtype != torch.float32:
-            raise ValueError("kernel must keep the plan's float32 kernel shape")
         self._kernel = kernel.contiguous()
         with torch.cuda.device(self.device):
             check(lib().fcamd_conv_plan_set_kernel(self._plan, self._kernel.data_ptr(),
@@ -235,11 +282,11 @@ class ConvolutionPlan:
     def run(self, image, out=None, stream=None):
         """Convolve one image of the plan's shape; ``out`` may be ``image``."""
         import torch
-        if tuple(image.shape) != self.image_shape or image.dtype != torch.float32:
-            raise ValueError("image must have the plan's shape and dtype float32")
+        _check_tensor(image, "image", self.image_shape, self.device, contiguous=False)
         image = image.contiguous()
         if out is None:
             out = torch.empty_like(image)
+        _check_tensor(out, "out", self.image_shape, self.device)
         with torch.cuda.device(self.device):
             check(lib().fcamd_conv_plan_execute(self._plan, out.data_ptr(), image.data_ptr(),
                                                 _stream_handle(stream)))
@@ -296,21 +343,17 @@ def convolve3d_boundary(image, kernel, boundary="zero", op=CONVOLVE, out=None, w
     ``image``.  The padding is virtual: no padded copy of the image is
     made."""
     import torch
-    if image.dtype != torch.float32 or kernel.dtype != torch.float32:
-        raise TypeError("float32 tensors required")
-    if not (image.is_cuda and kernel.is_cuda):
-        raise TypeError("CUDA tensors required (there is no CPU path)")
+    _check_tensor(image, "image", contiguous=False)
+    _check_tensor(kernel, "kernel", device=image.device, contiguous=False)
     if image.dim() != 3 or kernel.dim() != 3:
         raise ValueError("image and kernel must be 3D")
     image = image.contiguous()
     kernel = kernel.contiguous()
     if out is None:
         out = torch.empty_like(image)
-    elif out.shape != image.shape or not out.is_contiguous():
-        raise ValueError("out must be a contiguous tensor shaped like image")
-    wptr, wbytes = None, 0
-    if workspace is not None:
-        wptr, wbytes = workspace.data_ptr(), workspace.numel() * workspace.element_size()
+    _check_tensor(out, "out", image.shape, image.device)
+    need = boundary_workspace_bytes(image.shape, kernel.shape, boundary)
+    wptr, wbytes, _keep = _workspace(need, workspace, image.device, stream)
     with torch.cuda.device(image.device):
         check(lib().fcamd_convolve3d_boundary_device(
             out.data_ptr(), image.data_ptr(), ints(image.shape[::-1]),
@@ -322,25 +365,23 @@ def convolve3d_boundary(image, kernel, boundary="zero", op=CONVOLVE, out=None, w
 def convolve3d(image, kernel, op=CONVOLVE, out=None, workspace=None, stream=None):
     """Device-resident convolution of torch CUDA float32 tensors.
 
-    image [nz, ny, nx], kernel [kz, ky, kx] (contiguous, same device);
-    ``out`` may be ``image`` (in place).  Enqueued on ``stream`` (default:
-    torch's current stream); returns ``out``."""
+    image [nz, ny, nx], kernel [kz, ky, kx] (same device); ``out`` may be
+    ``image`` (in place).  ``workspace`` is a caller's buffer of
+    workspace_bytes() bytes, else one comes from torch's caching allocator.
+    Enqueued on ``stream`` (default: torch's current stream) without
+    synchronising; returns ``out``."""
     import torch
-    if image.dtype != torch.float32 or kernel.dtype != torch.float32:
-        raise TypeError("float32 tensors required")
-    if not (image.is_cuda and kernel.is_cuda):
-        raise TypeError("CUDA tensors required (there is no CPU path)")
+    _check_tensor(image, "image", contiguous=False)
+    _check_tensor(kernel, "kernel", device=image.device, contiguous=False)
     if image.dim() != 3 or kernel.dim() != 3:
         raise ValueError("image and kernel must be 3D")
     image = image.contiguous()
     kernel = kernel.contiguous()
     if out is None:
         out = torch.empty_like(image)
-    elif out.shape != image.shape or not out.is_contiguous():
-        raise ValueError("out must be a contiguous tensor shaped like image")
-    wptr, wbytes = None, 0
-    if workspace is not None:
-        wptr, wbytes = workspace.data_ptr(), workspace.numel() * workspace.element_size()
+    _check_tensor(out, "out", image.shape, image.device)
+    need = workspace_bytes(image.shape, kernel.shape)
+    wptr, wbytes, _keep = _workspace(need, workspace, image.device, stream)
     with torch.cuda.device(image.device):
         check(lib().fcamd_convolve3d_device(
             out.data_ptr(), image.data_ptr(), ints(image.shape[::-1]),

@@ -20,15 +20,16 @@ from ._lib import check, ints, lib
 def grad_kernel(grad_out, image, kernel_shape, workspace=None, stream=None):
     """dk[j] = sum_i g[i] * image[(i - (j - k//2)) mod n] on a kernel of
     C-order shape ``kernel_shape`` (torch CUDA float32)."""
-    from . import _stream_handle
-    if grad_out.shape != image.shape:
-        raise ValueError("grad_out must be shaped like image")
+    from . import _check_tensor, _stream_handle, _workspace
+    _check_tensor(image, "image", contiguous=False)
+    _check_tensor(grad_out, "grad_out", image.shape, image.device, contiguous=False)
+    if image.dim() != 3 or len(kernel_shape) != 3:
+        raise ValueError("image and kernel must be 3D")
     grad_out = grad_out.contiguous()
     image = image.contiguous()
     dk = torch.empty(tuple(kernel_shape), dtype=torch.float32, device=image.device)
-    wptr, wbytes = None, 0
-    if workspace is not None:
-        wptr, wbytes = workspace.data_ptr(), workspace.numel() * workspace.element_size()
+    need = grad_kernel_workspace_bytes(image.shape, kernel_shape)
+    wptr, wbytes, _keep = _workspace(need, workspace, image.device, stream)
     with torch.cuda.device(image.device):
         check(lib().fcamd_conv_grad_kernel_device(
             dk.data_ptr(), grad_out.data_ptr(), image.data_ptr(), ints(image.shape[::-1]),

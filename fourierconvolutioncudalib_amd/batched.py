@@ -15,7 +15,7 @@ import ctypes
 
 import torch
 
-from . import CONVOLVE, _stream_handle
+from . import CONVOLVE, _check_tensor, _stream_handle
 from ._lib import FCError, check, ints, lib
 
 
@@ -27,14 +27,20 @@ class BatchedConvolution:
     def __init__(self, images, kernels, outs=None, op=CONVOLVE):
         if len(images) != len(kernels) or not images:
             raise ValueError("need one kernel per image")
+        if outs is not None and len(outs) != len(images):
+            raise ValueError("need one output per image")
+        self.device = images[0].device
+        for i, (a, k) in enumerate(zip(images, kernels)):
+            _check_tensor(a, f"images[{i}]", device=self.device, contiguous=False)
+            _check_tensor(k, f"kernels[{i}]", device=self.device, contiguous=False)
+            if a.dim() != 3 or k.dim() != 3:
+                raise ValueError(f"block {i}: image and kernel must be 3D")
         self.images = [t.contiguous() for t in images]
         self.kernels = [t.contiguous() for t in kernels]
-        for t in self.images + self.kernels:
-            if t.dtype != torch.float32 or not t.is_cuda or t.dim() != 3:
-                raise TypeError("3D float32 CUDA tensors required")
         self.outs = outs if outs is not None else [torch.empty_like(t) for t in self.images]
+        for i, (o, a) in enumerate(zip(self.outs, self.images)):
+            _check_tensor(o, f"outs[{i}]", a.shape, self.device)
         n = len(self.images)
-        self.device = self.images[0].device
         imdims = ints([d for t in self.images for d in t.shape[::-1]])
         kdims = ints([d for t in self.kernels for d in t.shape[::-1]])
         ptrs = lambda ts: (ctypes.c_void_p * n)(*[t.data_ptr() for t in ts])  # noqa: E731

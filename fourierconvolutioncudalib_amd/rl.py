@@ -15,21 +15,22 @@ import ctypes
 
 import torch
 
-from . import _stream_handle
+from . import _check_tensor, _stream_handle, _workspace
 from ._lib import check, ints, lib
 
 
 def _args(views, kernels):
     if not views or len(views) != len(kernels):
         raise ValueError("need one PSF per view")
+    device = views[0].device
+    shape = tuple(views[0].shape)
+    for i, (v, k) in enumerate(zip(views, kernels)):
+        _check_tensor(v, f"views[{i}]", shape, device, contiguous=False)
+        _check_tensor(k, f"kernels[{i}]", device=device, contiguous=False)
+        if v.dim() != 3 or k.dim() != 3:
+            raise ValueError(f"view {i}: view and PSF must be 3D")
     views = [v.contiguous() for v in views]
     kernels = [k.contiguous() for k in kernels]
-    shape = tuple(views[0].shape)
-    for t in views + kernels:
-        if t.dtype != torch.float32 or not t.is_cuda or t.dim() != 3:
-            raise TypeError("3D float32 CUDA tensors required")
-    if any(tuple(v.shape) != shape for v in views):
-        raise ValueError("all views must share one shape")
     imdim = ints(shape[::-1])
     kdims = ints([d for k in kernels for d in k.shape[::-1]])
     return views, kernels, shape, imdim, kdims
@@ -49,17 +50,17 @@ def richardson_lucy(views, kernels, iterations=10, eps=1e-6, psi=None, fused=Tru
     init = psi is None
     if init:
         psi = torch.empty(shape, dtype=torch.float32, device=views[0].device)
-    elif tuple(psi.shape) != shape or not psi.is_contiguous():
-        raise ValueError("psi must be a contiguous tensor shaped like the views")
+    else:
+        _check_tensor(psi, "psi", shape, views[0].device)
     vptr = (ctypes.c_void_p * n)(*[v.data_ptr() for v in views])
     kptr = (ctypes.c_void_p * n)(*[k.data_ptr() for k in kernels])
     with torch.cuda.device(psi.device):
         nbytes = lib().fcamd_rl_workspace_bytes(n, imdim, kdims, 1 if fused else 0)
         if nbytes < 0:
             check(lib().fcamd_last_error())
-        ws = torch.empty(nbytes, dtype=torch.uint8, device=psi.device)
+        wptr, wbytes, _keep = _workspace(nbytes, None, psi.device, stream)
         check(lib().fcamd_rl_multiview_device(psi.data_ptr(), vptr, n, imdim, kptr, kdims,
                                               int(iterations), float(eps), 1 if init else 0,
-                                              1 if fused else 0, ws.data_ptr(), nbytes,
+                                              1 if fused else 0, wptr, wbytes,
                                               _stream_handle(stream)))
     return psi

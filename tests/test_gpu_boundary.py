@@ -105,5 +105,12 @@ def test_boundary_host_entry_and_workspace(fc, oracle):
     got = fc.convolve3d_boundary(ta, torch.from_numpy(k).cuda(), "mirror", out=ta, workspace=ws)
     assert oracle.rel_l2(got.cpu().numpy(), ref) < TOL
     small = torch.empty(nbytes - 256, dtype=torch.uint8, device="cuda")
-    with pytest.raises(fc.FCError):
-        fc.convolve3d_boundary(ta, torch.from_numpy(k).cuda(), "mirror", workspace=small)
+    tk = torch.from_numpy(k).cuda()
+    # the Python wrapper refuses a short workspace before calling the library ...
+    with pytest.raises(ValueError, match="workspace"):
+        fc.convolve3d_boundary(ta, tk, "mirror", workspace=small)
+    # ... and so does the C entry point itself
+    rc = fc.lib().fcamd_convolve3d_boundary_device(
+        ta.data_ptr(), ta.data_ptr(), ints(a.shape[::-1]), tk.data_ptr(), ints(k.shape[::-1]),
+        fc.CONVOLVE, fc.BOUNDARY["mirror"], small.data_ptr(), nbytes - 256, None)
+    assert rc == 1 and b"workspace" in fc.lib().fcamd_last_error_message()

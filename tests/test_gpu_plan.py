@@ -42,8 +42,13 @@ def test_plan_kernel_change_correlate_workspace_inplace(fc):
     plan.run(b, out=b)
     assert torch.equal(b, fc.convolve3d(a, k2, op=fc.CORRELATE))
     small = torch.empty(ws.numel() - 256, dtype=torch.uint8, device="cuda")
-    with pytest.raises(fc.FCError):
+    with pytest.raises(ValueError, match="workspace"):  # the wrapper's check
         fc.ConvolutionPlan(shape, k1, workspace=small)
+    from fourierconvolutioncudalib_amd._lib import ints
+    assert not fc.lib().fcamd_conv_plan_create(  # the C entry point's own check
+        ints(shape[::-1]), ints(k1.shape[::-1]), k1.data_ptr(), 0, small.data_ptr(),
+        small.numel(), None)
+    assert fc.lib().fcamd_last_error() == 1
     with pytest.raises(ValueError):
         plan.run(torch.zeros((40, 64, 71), device="cuda"))
     plan.close()
