@@ -112,6 +112,12 @@ float2* stage_table(int dev, int L) {
   return d;
 }
 
+bool tables_cached(int dev, int L) {
+  std::lock_guard<std::mutex> lk(g_tw_mu);
+  const auto key = std::make_pair(dev, L);
+  return g_tw.count(key) != 0 && g_stw.count(key) != 0;
+}
+
 int current_device() {
   int dev = 0;
   FC_HIP_CHECK(hipGetDevice(&dev));
@@ -122,7 +128,7 @@ std::once_flag g_attr_once;
 
 }  // namespace
 
-FftDesc fft_desc(int L) {
+FftDesc fft_desc(int L, hipStream_t st) {
   FftDesc d{};
   d.L = L;
   std::vector<int> r = radix_plan(L);
@@ -130,6 +136,16 @@ FftDesc fft_desc(int L) {
   d.nst = (int)r.size();
   for (int i = 0; i < d.nst; ++i) d.rad[i] = r[i];
   const int dev = current_device();
+  if (!tables_cached(dev, L) && st != nullptr) {
+    // creating the tables allocates and copies synchronously, which a
+    // stream capture cannot record: refuse instead of breaking the capture
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    FC_HIP_CHECK(hipStreamIsCapturing(st, &cs));
+    FC_REQUIRE(cs == hipStreamCaptureStatusNone, kInvalidArgument,
+               "transform length " + std::to_string(L) +
+                   " is used for the first time inside a stream capture; run one call of "
+                   "this shape (or create a plan) before capturing");
+  }
   d.tw = twiddle_table(dev, L);
   d.stw = stage_table(dev, L);
   return d;
@@ -296,7 +312,7 @@ void pass_x_forward(const ConvShape& s, const float* src, float2* S, const float
   ensure_kernel_attributes();
   const int nb = half_bins(s.nx);
   const long nxp = padded_bins(s.nx);
-  const FftDesc fx = fft_desc(s.nx);
+  const FftDesc fx = fft_desc(s.nx, st);
   const long nrows = (long)s.ny * s.nz;
   // x-R2C of the image rows (and of the kernel rows, in the same launch
   // when a compile-time kernel serves nx)
@@ -335,7 +351,7 @@ void pass_y_forward(const ConvShape& s, float2* S, const float2* Kx, float2* Kxy
   ensure_kernel_attributes();
   const int nb = half_bins(s.nx);
   const long nxp = padded_bins(s.nx);
-  const FftDesc fy = fft_desc(s.ny);
+  const FftDesc fy = fft_desc(s.ny, st);
   const bool kernel = Kx != nullptr;
   // y-FFT of the spectrum (and kernel rows -> kernel planes)
   {
@@ -374,7 +390,7 @@ void z_pass(const ConvShape& s, float2* S, const float2* operand, bool dense, bo
   const long nxp = padded_bins(s.nx);
   PassScope ps(kPassZFused, st);
   ZArgs a{};
-  a.fft = fft_desc(s.nz);
+  a.fft = fft_desc(s.nz, st);
   a.tx = col_tile(s.nz, 3);
   a.nbins = half_bins(s.nx);
   a.s = S;
@@ -392,7 +408,7 @@ void z_pass(const ConvShape& s, float2* S, const float2* operand, bool dense, bo
 
 void pass_y_inverse(const ConvShape& s, float2* S, hipStream_t st) {
   ensure_kernel_attributes();
-  const FftDesc fy = fft_desc(s.ny);
+  const FftDesc fy = fft_desc(s.ny, st);
   if (s.ny > 1) {
     PassScope ps(kPassYInverse, st);
     if (!static_ycol(s, fy, S, +1, nullptr, nullptr, st))
@@ -408,7 +424,7 @@ void inverse_yx(const ConvShape& s, float2* S, float* dst, hipStream_t st, const
 void pass_x_inverse(const ConvShape& s, const float2* S, float* dst, hipStream_t st,
                     const VPad* vp) {
   ensure_kernel_attributes();
-  const FftDesc fx = fft_desc(s.nx);
+  const FftDesc fx = fft_desc(s.nx, st);
   PassScope ps(kPassXInverse, st);
   if (!static_xinv(s, fx, S, dst, st, vp)) {
     const long nrows = (long)s.ny * s.nz;

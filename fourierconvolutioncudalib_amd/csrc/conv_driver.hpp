@@ -37,8 +37,10 @@ void validate_shape(const ConvShape& s);
 // One-time kernel attribute set-up (safe to call repeatedly).
 void driver_init();
 
-// FFT description for length L on the current device (tables cached).
-FftDesc fft_desc(int L);
+// FFT description for length L on the current device.  Its twiddle tables
+// are created on first use and cached for the process; with `st` given, a
+// first use while `st` is capturing a graph is refused (kInvalidArgument).
+FftDesc fft_desc(int L, hipStream_t st = nullptr);
 
 // Radix plan used for length L.
 std::vector<int> radix_plan(int L);
