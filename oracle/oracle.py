@@ -250,13 +250,107 @@ def max_abs(a, ref):
     return float(np.max(np.abs(np.asarray(a, np.float64) - np.asarray(ref, np.float64))))
 
 
-def numpy_fft_conv(image, kernel, conj_kernel=False):
-    """FP64 FFT circular convolution via numpy's pocketfft (for shapes where
-    the C oracle's O(n log n) recursion is too slow); pinned against
-    fft_conv() on small shapes by the oracle tests."""
+def fft_workers():
+    """Threads for the FP64 FFTs below: the host cores this process may use,
+    at most 16 (one GPU's share of the host)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def _rfftn(a):
+    try:
+        import scipy.fft as sf
+        return sf.rfftn(a, workers=fft_workers())
+    except ImportError:
+        return np.fft.rfftn(a)
+
+
+def _irfftn(A, shape):
+    axes = tuple(range(len(shape)))
+    try:
+        import scipy.fft as sf
+        return sf.irfftn(A, s=shape, axes=axes, workers=fft_workers())
+    except ImportError:
+        return np.fft.irfftn(A, s=shape, axes=axes)
+
+
+def kernel_spectrum(kernel, image_shape, conj_kernel=False):
+    """FP64 half spectrum of the kernel zero-padded and centred at k/2 on
+    the image grid (src/convolution3Dfft.cu:128-166, x-fastest roles)."""
+    K = _rfftn(fftshift_pad(kernel, image_shape))
+    return np.conj(K) if conj_kernel else K
+
+
+def numpy_fft_conv(image, kernel, conj_kernel=False, K=None):
+    """FP64 FFT circular convolution via pocketfft (scipy.fft, threaded;
+    numpy.fft when scipy is absent), for shapes where the C oracle's O(n log n)
+    recursion is too slow; pinned against fft_conv() and the direct form on
+    small shapes by the oracle tests.  ``K`` is a precomputed
+    kernel_spectrum()."""
     image = np.asarray(image, dtype=np.float64)
-    pk = fftshift_pad(kernel, image.shape)
-    K = np.fft.rfftn(pk)
-    if conj_kernel:
-        K = np.conj(K)
-    return np.fft.irfftn(np.fft.rfftn(image) * K, s=image.shape)
+    if K is None:
+        K = kernel_spectrum(kernel, image.shape, conj_kernel)
+    return _irfftn(_rfftn(image) * K, image.shape)
+
+
+# numpy.pad modes of the boundary rules (half-sample symmetric is numpy's
+# 'symmetric', whole-sample symmetric its 'reflect')
+_NP_PAD = {1: "constant", 2: "edge", 3: "symmetric", 4: "reflect"}
+
+
+def fft_boundary_conv(image, kernel, mode, correlate=False):
+    """FP64 boundary-mode convolution by FFT: the image extended by rule
+    ``mode`` (a BOUNDARY name or number other than periodic) far enough that
+    no interior output wraps, circularly convolved on that grid, cropped.
+    The FFT form of boundary_conv_direct() (pinned against it by the oracle
+    tests) for sizes where the direct sum is too slow."""
+    m = BOUNDARY[mode] if isinstance(mode, str) else int(mode)
+    if m not in _NP_PAD:
+        raise ValueError("periodic is numpy_fft_conv")
+    image = np.asarray(image, dtype=np.float64)
+    k = np.asarray(kernel, dtype=np.float64)
+    if correlate:
+        # correlation is convolution with the kernel flipped about its centre
+        # k/2: tap j moves to 2(k/2) - j, which needs an odd extent per axis
+        for d, n in enumerate(k.shape):
+            if n % 2 == 0:
+                k = np.concatenate([k, np.zeros_like(k.take([0], axis=d))], axis=d)
+        k = k[::-1, ::-1, ::-1]
+    pads = [(n_k - 1 - n_k // 2, n_k // 2) for n_k in k.shape]
+    padded = np.pad(image, pads, mode=_NP_PAD[m])
+    full = numpy_fft_conv(padded, k.astype(np.float32) if k.dtype != np.float32 else k)
+    sl = tuple(slice(lo, lo + n) for (lo, _), n in zip(pads, image.shape))
+    return full[sl]
+
+
+def fft_conv_grad_kernel(g, image, kernel_shape):
+    """FP64 gradient of sum(g * conv(image, k)) with respect to the kernel
+    taps by FFT: u = IFFT(FFT(g) conj(FFT(image))), dk[j] = u[(j - k/2) mod n];
+    pinned against the direct conv_grad_kernel() by the oracle tests."""
+    g = np.asarray(g, dtype=np.float64)
+    im = np.asarray(image, dtype=np.float64)
+    u = _irfftn(_rfftn(g) * np.conj(_rfftn(im)), g.shape)
+    idx = [(np.arange(k) - k // 2) % n for k, n in zip(kernel_shape, g.shape)]
+    return u[np.ix_(*idx)]
+
+
+def fft_rl_multiview(views, kernels, iterations, eps=1e-6):
+    """FP64 multiview Richardson-Lucy by FFT (psi0 = mean of the views):
+    the update of rl_multiview(), with each PSF's spectrum computed once;
+    pinned against rl_multiview() by the oracle tests."""
+    views = [np.asarray(v, dtype=np.float64) for v in views]
+    shape = views[0].shape
+    Ks = [kernel_spectrum(k, shape) for k in kernels]
+    psi = np.mean(views, axis=0)
+    for _ in range(int(iterations)):
+        P = _rfftn(psi)
+        acc = None
+        for v, K in zip(views, Ks):
+            blur = _irfftn(P * K, shape)
+            R = _rfftn(v / np.maximum(blur, eps)) * np.conj(K)
+            acc = R if acc is None else acc + R
+        psi = psi * _irfftn(acc, shape) / len(views)
+    return psi

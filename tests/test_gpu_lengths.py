@@ -77,29 +77,82 @@ def test_correlate_static(fc, oracle):
     assert oracle.rel_l2(out, oracle.numpy_fft_conv(img, k, conj_kernel=True)) < TOL
 
 
-@pytest.mark.parametrize("L", [512, 256, 600, 105, 64, 1024])
-def test_every_variant(fc, oracle, L):
-    """Each compile-time variant of each pass, selected in turn, against the
-    FP64 oracle (small and large kz for the fused z pass)."""
+# Kernel extents of the timed workloads along the transformed axis (15, 21,
+# 25, 31, 41, 51, 61) and a short one: the fused z pass picks its tap path,
+# the general FFT path, the direct DFT or that variant's fallback from the
+# kernel's z extent, so every (length, variant) runs at each extent a
+# workload can give it.
+KERNEL_EXTENTS = (3, 15, 21, 25, 31, 41, 51, 61)
+PASS_NAMES = {0: "x forward", 1: "x inverse", 2: "y", 3: "fused z"}
+
+
+def _variant_cases(p, L):
+    """(image shape, kernel shape) pairs, C order, that put length L on the
+    axis pass p transforms."""
+    ext = [k for k in KERNEL_EXTENTS if k <= L]
+    if p in (0, 1):
+        return [((6, 10, L), (3, 5, k)) for k in (ext[0], ext[-1])]
+    if p == 2:
+        return [((6, L, 24), (3, k, 5)) for k in (ext[0], ext[-1])]
+    return [((L, 6, 18), (k, 5, 3)) for k in ext]
+
+
+@pytest.mark.parametrize("p", [0, 1, 2, 3])
+def test_every_variant(fc, oracle, p):
+    """Every compile-time variant of pass p at every registered length,
+    selected in turn, against the FP64 oracle: whatever the autotuner picks
+    for a timed workload has run here at that workload's length and kernel
+    extent."""
+    from fourierconvolutioncudalib_amd import synth
     lib = fc.lib()
-    cases = {0: [(6, 10, L)], 1: [(6, 10, L)], 2: [(6, L, 24)],
-             3: [(L, 10, 24), (L, 6, 18)]}
-    prev = {p: lib.fcamd_selected_static_variant(p, L) for p in cases}
-    try:
-        for p, shapes in cases.items():
+    checked = 0
+    for L in _lengths():
+        prev = lib.fcamd_selected_static_variant(p, L)
+        try:
             n = lib.fcamd_static_variant_count(p, L)
             assert n >= 1
             for v in range(n):
                 assert lib.fcamd_select_static_variant(p, L, v) == 0
-                for si, shape in enumerate(shapes):
-                    if si == 0:
-                        kshape = tuple(min(s, k) for s, k in zip(shape, (5, 7, 3)))
-                    else:  # kz beyond the direct-DFT limit -> fallback variant
-                        kshape = (min(L, 61 if L >= 128 else L - 1), 5, 3)
-                    _check(fc, oracle, shape, kshape, 10 * p + v)
+                name = lib.fcamd_static_variant_name(p, L, v).decode()
+                for shape, kshape in _variant_cases(p, L):
+                    img = synth.uniform(shape, 1000 * p + L + v)
+                    k = synth.random_psf(kshape, 7 * v + kshape[0])
+                    err = oracle.rel_l2(fc.convolve(img, k), oracle.numpy_fft_conv(img, k))
+                    assert err < TOL, (PASS_NAMES[p], L, name, shape, kshape, err)
+                    checked += 1
+        finally:
+            lib.fcamd_select_static_variant(p, L, prev)
+    print(f"{PASS_NAMES[p]}: {checked} (length, variant, kernel) cases", flush=True)
+
+
+@pytest.mark.parametrize("cfg", ["METRIC", "CFG1", "CFG2"])
+def test_tuned_workloads_full_size(fc, oracle, cfg):
+    """The plain workloads the benchmark times, at full size, with the
+    variants the autotuner selects for them (as the benchmark does before
+    timing), against the FP64 oracle."""
+    import torch
+    from fourierconvolutioncudalib_amd import synth
+    lib = fc.lib()
+    w = getattr(synth, cfg)
+    img, psf = synth.workload(w)
+    nz, ny, nx = img.shape
+    keep = {(q, L): lib.fcamd_selected_static_variant(q, L)
+            for q, L in ((0, nx), (1, nx), (2, ny), (3, nz))}
+    print(cfg, img.shape, psf.shape, flush=True)
+    try:
+        ti = torch.from_numpy(img).cuda()
+        tk = torch.from_numpy(psf).cuda()
+        fc.autotune(ti.shape, tk.shape, reps=2)
+        tuned = fc.selected_variants(ti.shape)
+        out = fc.convolve3d(ti, tk).cpu().numpy()
     finally:
-        for p in cases:
-            lib.fcamd_select_static_variant(p, L, prev[p])
+        for (q, L), v in keep.items():
+            if v >= 0:
+                lib.fcamd_select_static_variant(q, L, v)
+    ref = oracle.numpy_fft_conv(img, psf)
+    err = oracle.rel_l2(out, ref)
+    print(cfg, tuned, f"rel-L2 {err:.2e} max-abs {oracle.max_abs(out, ref):.2e}", flush=True)
+    assert err < TOL, (cfg, tuned, err)
 
 
 @pytest.mark.parametrize("L", [64, 105, 128, 210, 240, 512])

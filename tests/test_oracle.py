@@ -227,6 +227,19 @@ def test_rl_flux_conservation_and_numpy_restatement(oracle):
             acc += oracle.numpy_fft_conv(v / np.maximum(blur, 1e-6), k, conj_kernel=True)
         p = p * acc / len(ks)
     assert oracle.rel_l2(psi, p) < 1e-12
+    # the FFT form used at full size
+    assert oracle.rel_l2(oracle.fft_rl_multiview(views, ks, 3), psi) < 1e-12
+
+
+@pytest.mark.parametrize("shape,kshape", [((6, 7, 8), (3, 5, 3)), ((9, 4, 5), (5, 3, 1)),
+                                          ((5, 5, 6), (5, 5, 6))])
+def test_fft_grad_kernel_vs_direct(oracle, shape, kshape):
+    g = _rng(31).random(shape)
+    img = _rng(32).random(shape, dtype=np.float32)
+    d = oracle.conv_grad_kernel(g, img, kshape)
+    f = oracle.fft_conv_grad_kernel(g, img, kshape)
+    assert f.shape == tuple(kshape)
+    assert oracle.rel_l2(f, d) < 1e-13
 
 
 # ---------------------------------------------------------------------------
@@ -253,6 +266,22 @@ def test_boundary_oracle_matches_scipy_ndimage(oracle, shape, kshape, mode):
                          cval=0.0)
     gotc = oracle.boundary_conv_direct(a, k, mode, correlate=True)
     assert np.abs(gotc - refc).max() <= 1e-12 * max(1.0, np.abs(refc).max())
+
+
+@pytest.mark.parametrize("shape,kshape", [((5, 6, 7), (3, 3, 3)), ((4, 9, 6), (5, 3, 7)),
+                                          ((3, 4, 5), (7, 9, 3)), ((6, 5, 8), (4, 2, 3))])
+@pytest.mark.parametrize("mode", ["zero", "nearest", "reflect", "mirror"])
+def test_fft_boundary_form_vs_direct(oracle, shape, kshape, mode):
+    """The FFT form used at full size against the direct form pinned above,
+    for convolution and correlation, odd and even kernels, kernels larger
+    than the image."""
+    rng = np.random.default_rng(sum(shape) * 3 + sum(kshape))
+    a = rng.random(shape).astype(np.float32)
+    k = rng.random(kshape).astype(np.float32)
+    for corr in (False, True):
+        d = oracle.boundary_conv_direct(a, k, mode, correlate=corr)
+        f = oracle.fft_boundary_conv(a, k, mode, correlate=corr)
+        assert oracle.rel_l2(f, d) < 1e-13, (mode, corr)
 
 
 def test_boundary_oracle_closed_forms(oracle):
