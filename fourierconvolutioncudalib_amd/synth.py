@@ -7,6 +7,8 @@ descriptions.
 """
 from __future__ import annotations
 
+import os
+
 import numpy as np
 
 _G = np.uint64(0x9E3779B97F4A7C15)
@@ -137,7 +139,15 @@ def multiview(shape_zyx, kshape_zyx, nviews: int, seed: int, sigma_xyz=(2.0, 2.0
     phantom = blobs(shape_zyx, seed, count=blobs_count, background=10.0, poisson=False)
     psfs = [gaussian_psf(kshape_zyx, sigma_xyz, rotation_y(180.0 * v / nviews))
             for v in range(nviews)]
-    fp = np.fft.rfftn(phantom.astype(np.float64))
+    try:  # threaded pocketfft when scipy is there (input generation only)
+        import scipy.fft as sf
+        workers = max(1, min(16, len(os.sched_getaffinity(0))))
+        rfftn = lambda a: sf.rfftn(a, workers=workers)  # noqa: E731
+        irfftn = lambda a, s: sf.irfftn(a, s=s, axes=(0, 1, 2), workers=workers)  # noqa: E731
+    except ImportError:
+        rfftn = np.fft.rfftn
+        irfftn = lambda a, s: np.fft.irfftn(a, s=s, axes=(0, 1, 2))  # noqa: E731
+    fp = rfftn(phantom.astype(np.float64))
     rng = np.random.default_rng(seed + 1)
     views = []
     for k in psfs:
@@ -147,7 +157,7 @@ def multiview(shape_zyx, kshape_zyx, nviews: int, seed: int, sigma_xyz=(2.0, 2.0
         yi = (np.arange(ky) - ky // 2) % shape_zyx[1]
         xi = (np.arange(kx) - kx // 2) % shape_zyx[2]
         pk[np.ix_(zi, yi, xi)] = k
-        blur = np.fft.irfftn(fp * np.fft.rfftn(pk), s=shape_zyx, axes=(0, 1, 2))
+        blur = irfftn(fp * rfftn(pk), shape_zyx)
         blur = np.maximum(blur, 0.0)
         views.append((rng.poisson(blur) if noise else blur).astype(np.float32))
     return phantom, views, psfs
