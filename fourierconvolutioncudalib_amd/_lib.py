@@ -41,6 +41,7 @@ SIGNATURES = {
     "fcamd_free_result": (None, [_fp]),
     "fcamd_convolve3d_host": (_i, [_fp, _ip, _fp, _ip, _i, _i]),
     "fcamd_workspace_bytes": (_ll, [_ip, _ip]),
+    "fcamd_host_device_bytes": (_ll, [_ip, _ip, _i]),
     "fcamd_convolve3d_device": (_i, [_vp, _vp, _ip, _vp, _ip, _i, _vp, _ll, _vp]),
     "fcamd_radix_plan": (_i, [_i, _ip, _i]),
     "fcamd_grad_kernel_workspace_bytes": (_ll, [_ip, _ip]),

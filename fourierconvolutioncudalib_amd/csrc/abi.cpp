@@ -99,6 +99,50 @@ bool slabs_enabled() {
   return on;
 }
 
+// Device memory the host-pointer path may use on `dev`: FCAMD_DEVICE_BUDGET_MB
+// when set (read on every call), else the device's free memory plus what the
+// pool already holds, less a 256 MiB margin.
+bool env_budget(size_t* bytes) {
+  const char* v = std::getenv("FCAMD_DEVICE_BUDGET_MB");
+  if (!v || !v[0]) return false;
+  const long long mb = std::atoll(v);
+  FC_REQUIRE(mb > 0, kInvalidArgument, "FCAMD_DEVICE_BUDGET_MB must be a positive integer");
+  *bytes = (size_t)mb << 20;
+  return true;
+}
+
+size_t device_budget(const DevicePool& pool) {
+  size_t b = 0;
+  if (env_budget(&b)) return b;
+  size_t free_b = 0, total_b = 0;
+  FC_HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
+  const size_t held = pool.img_bytes + pool.out_bytes + pool.ws_bytes + pool.ker_bytes;
+  const size_t avail = free_b + held, margin = size_t(256) << 20;
+  return avail > margin ? avail - margin : 0;
+}
+
+// How a host-pointer convolution of `s` runs within `budget`: a z-slab plan
+// (nslabs > 0) or the single shot.
+struct HostPlan {
+  SlabPlan sp;
+  size_t device_bytes = 0;  // everything it allocates on the device
+};
+
+HostPlan host_plan(const ConvShape& s, size_t budget) {
+  HostPlan h;
+  const size_t single = single_shot_device_bytes(s);
+  const size_t nker = (sizeof(float) * (size_t)s.kx * s.ky * s.kz + 255) / 256 * 256;
+  if (slabs_enabled()) {
+    h.sp = slab_plan(s, budget > nker ? budget - nker : 0, single > budget);
+    if (h.sp.nslabs > 0) {
+      h.device_bytes = h.sp.device_bytes() + nker;
+      return h;
+    }
+  }
+  h.device_bytes = single;
+  return h;
+}
+
 void host_convolution(const ConvShape& s, const float* im, const float* kernel, float* out,
                       Op op, int dev) {
   FC_REQUIRE(im != nullptr && kernel != nullptr && out != nullptr, kInvalidArgument,
@@ -109,18 +153,26 @@ void host_convolution(const ConvShape& s, const float* im, const float* kernel, 
   if (!pool.stream) FC_HIP_CHECK(hipStreamCreateWithFlags(&pool.stream, hipStreamNonBlocking));
   const size_t nimg = sizeof(float) * (size_t)s.voxels();
   const size_t nker = sizeof(float) * (size_t)s.kx * s.ky * s.kz;
-  const SlabPlan sp = slab_plan(s);
-  if (sp.nslabs > 0 && slabs_enabled()) {
-    // large volumes: z-slab streaming, both PCIe directions at once
-    void* din = pool_reserve(reinterpret_cast<void**>(&pool.img), &pool.img_bytes, nimg);
-    void* dout = pool_reserve(reinterpret_cast<void**>(&pool.out), &pool.out_bytes, nimg);
+  const size_t budget = device_budget(pool);
+  const HostPlan hp = host_plan(s, budget);
+  if (hp.sp.nslabs > 0) {
+    // z-slab streaming through a ring of slab buffers, both PCIe directions at once
+    const SlabPlan& sp = hp.sp;
+    void* ring = pool_reserve(reinterpret_cast<void**>(&pool.img), &pool.img_bytes,
+                              sp.ring_bytes);
+    void* wrap = pool_reserve(reinterpret_cast<void**>(&pool.out), &pool.out_bytes,
+                              sp.wrap_bytes);
     void* dker = pool_reserve(reinterpret_cast<void**>(&pool.ker), &pool.ker_bytes, nker);
-    void* ws = pool_reserve(&pool.ws, &pool.ws_bytes, slab_workspace_bytes(s, sp));
-    host_convolution_slabs(s, sp, im, kernel, out, op, dev, static_cast<float*>(din),
-                           static_cast<float*>(dout), static_cast<float*>(dker), ws, pool.stream,
+    void* ws = pool_reserve(&pool.ws, &pool.ws_bytes, sp.ws_bytes);
+    host_convolution_slabs(s, sp, im, kernel, out, op, dev, static_cast<float*>(ring),
+                           static_cast<float*>(wrap), static_cast<float*>(dker), ws, pool.stream,
                            &host_copy_parallel);
     return;
   }
+  FC_REQUIRE(hp.device_bytes <= budget, kOutOfMemory,
+             "the convolution needs " + std::to_string(hp.device_bytes >> 20) +
+                 " MiB of device memory (device budget " + std::to_string(budget >> 20) +
+                 " MiB) and no z-slab plan fits the budget");
   const size_t nws = workspace_layout(s).total();
   void* dimg = pool_reserve(reinterpret_cast<void**>(&pool.img), &pool.img_bytes, nimg);
   void* dker = pool_reserve(reinterpret_cast<void**>(&pool.ker), &pool.ker_bytes, nker);
@@ -130,6 +182,16 @@ void host_convolution(const ConvShape& s, const float* im, const float* kernel, 
   enqueue_convolution(s, static_cast<float*>(dimg), static_cast<float*>(dimg),
                       static_cast<float*>(dker), op, ws, pool.stream);
   staged_download(dev, out, dimg, nimg, pool.stream);
+}
+
+// device bytes a host-pointer convolution of `s` would allocate on `dev` now
+long long host_device_bytes(const ConvShape& s, int dev) {
+  size_t b = 0;
+  if (env_budget(&b)) return (long long)host_plan(s, b).device_bytes;  // no device needed
+  DeviceGuard guard(dev);
+  DevicePool& pool = device_pool(dev);
+  std::lock_guard<std::mutex> lk(pool.mu);
+  return (long long)host_plan(s, device_budget(pool)).device_bytes;
 }
 
 // host-pointer boundary-mode convolution through the device pool
@@ -223,18 +285,29 @@ imageType* convolution3DfftCUDA_test(imageType* im, int* imDim, imageType* kerne
 // src/convolution3Dfft.cu:581-584
 int cuda_version() { return HIP_VERSION; }
 
-// src/convolution3Dfft.cu:587-603 (cufftEstimate*d): the half-spectrum
-// workspace of an R2C transform of `shape` (last entry contiguous), in MiB.
+// src/convolution3Dfft.cu:587-603 (cufftEstimate*d, the FFT work area of the
+// reference's path).  Here: the device memory a host-pointer convolution of
+// an image of `shape` (len 1..3, last entry contiguous) allocates on the
+// current device under its device budget (z-slab ring or single shot), for a
+// one-voxel kernel, in MiB rounded up; fcamd_host_device_bytes gives the
+// figure for a given kernel.  -1 for an invalid request.
 int gpu_mem_needed_mb(int* shape, int len) {
   if (shape == nullptr || len < 1 || len > 3) return -1;
-  long long outer = 1;
-  for (int i = 0; i < len - 1; ++i) {
+  int d[3] = {1, 1, 1};  // {x, y, z}
+  for (int i = 0; i < len; ++i) {
     if (shape[i] < 1) return -1;
-    outer *= shape[i];
+    d[len - 1 - i] = shape[i];
   }
-  if (shape[len - 1] < 1) return -1;
-  const long long bytes = outer * padded_bins(shape[len - 1]) * (long long)sizeof(float2);
-  return (int)(bytes / (1 << 20));
+  const int k[3] = {1, 1, 1};
+  long long b = -1;
+  const int rc = guarded([&] {
+    int dev = 0;
+    size_t env = 0;
+    if (!env_budget(&env)) FC_HIP_CHECK(hipGetDevice(&dev));
+    b = host_device_bytes(shape_from(d, k), dev);
+  });
+  if (rc != kOk || b < 0) return -1;
+  return (int)((b + (1 << 20) - 1) >> 20);
 }
 
 // ===========================================================================
@@ -305,6 +378,12 @@ int fcamd_convolve3d_host(float* im, const int* imDim, const float* kernel, cons
     ConvShape s = shape_from(imDim, kernelDim);
     host_convolution(s, im, kernel, im, op_from(op), devCUDA);
   });
+}
+
+long long fcamd_host_device_bytes(const int* imDim, const int* kernelDim, int devCUDA) {
+  long long r = -1;
+  (void)guarded([&] { r = host_device_bytes(shape_from(imDim, kernelDim), devCUDA); });
+  return r;
 }
 
 long long fcamd_workspace_bytes(const int* imDim, const int* kernelDim) {

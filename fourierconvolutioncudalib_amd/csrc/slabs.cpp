@@ -1,16 +1,27 @@
 // slabs.cpp -- host-pointer convolution streamed through the device in
-// z-slabs (overlap-save).
+// z-slabs (overlap-save), holding only a ring of slab buffers.
 //
-// The output planes are cut into slabs of m planes.  Slab [z0, z0 + m) is
-// the circular convolution of a virtually padded problem whose z axis reads
-// the full volume's planes (z0 - lo + p) mod nz for p < P = fast length
-// >= m + kz - 1 (lo = kz - 1 - kz/2), so its interior equals the circular
-// convolution of the whole volume there; x and y are not padded.  The host
-// volume is uploaded in plane chunks (tail planes needed by the first slab
-// first), each slab's convolution starts as soon as its planes are on the
-// device, and finished slabs are downloaded while later planes are still
-// being uploaded: the two PCIe directions, the host-side staging copies and
-// the device work overlap.  All transfers go through pinned buffers.
+// The output planes are cut into slabs of m planes.  Slab [z0, z0 + m) needs
+// the input window of planes (z0 - lo + p) mod nz, p < W = m + kz - 1
+// (lo = kz - 1 - kz/2): its circular convolution, transformed at a z length
+// P >= W with the window's planes beyond W reading zero, equals the circular
+// convolution of the whole volume on the slab's m planes.  x and y are not
+// padded.  So the device never holds a whole volume:
+//   * a ring of kRingBuffers window buffers of W planes: slab k's window
+//     goes into buffer k mod kRingBuffers.  Consecutive windows overlap by
+//     kz - 1 planes (the halo), which are copied on the device from the
+//     previous slab's buffer; only the m new planes cross PCIe.
+//   * the slab's output is written into planes [0, m) of its own buffer
+//     (the x-C2R runs after the x-R2C has read the window), leaving the
+//     halo planes [m, W) intact for the next slab, and is downloaded from
+//     there.
+//   * the last slab's window wraps to planes [0, kz/2); those are copied
+//     into a small wrap buffer from slab 0's window before slab 0's output
+//     overwrites them, because an in-place call writes the first slab's
+//     output over the host planes before the last slab is uploaded.
+// Uploads, slab convolutions and downloads run on three streams driven by
+// one host loop: while slab k computes, slab k+1 uploads and slab k-1
+// downloads.  All transfers go through pinned buffers.
 #include "slabs.hpp"
 
 #include <algorithm>
@@ -26,6 +37,8 @@ namespace {
 
 constexpr size_t kSlabChunk = size_t(8) << 20;  // target bytes per transfer chunk
 constexpr int kUpBufs = 4, kDownBufs = 4;
+
+size_t align256(size_t b) { return (b + 255) / 256 * 256; }
 
 struct SlabStreams {
   hipStream_t up = nullptr, down = nullptr;
@@ -83,33 +96,47 @@ bool done(hipEvent_t e) {
   return false;
 }
 
+size_t plane_bytes(const ConvShape& s) { return sizeof(float) * (size_t)s.nx * s.ny; }
+
+// buffer sizes of a slab plan with output slabs of m planes at length P
+void size_plan(const ConvShape& s, SlabPlan& p) {
+  const size_t W = (size_t)p.m + s.kz - 1;
+  p.ring_bytes = kRingBuffers * align256(W * plane_bytes(s));
+  p.wrap_bytes = align256(std::max<size_t>(1, (size_t)s.cz) * plane_bytes(s));
+  p.ws_bytes = workspace_layout(slab_window_problem(s, p.m, p.P).padded).total();
+}
+
 }  // namespace
 
-SlabPlan slab_plan(const ConvShape& s) {
-  SlabPlan p{};
-  const size_t plane = sizeof(float) * (size_t)s.nx * s.ny;
-  const size_t bytes = plane * (size_t)s.nz;
-  if (bytes < kSlabMinBytes || plane > kSlabChunk || s.kz < 1) return p;
-  // largest compile-time z length P with m = P - (kz - 1) >= 2 (kz - 1) and
-  // at least four slabs
+size_t single_shot_device_bytes(const ConvShape& s) {
+  return align256(sizeof(float) * (size_t)s.voxels()) + workspace_layout(s).total() +
+         align256(sizeof(float) * (size_t)s.kx * s.ky * s.kz);
+}
+
+SlabPlan slab_plan(const ConvShape& s, size_t budget, bool force) {
+  SlabPlan best{};
+  const size_t bytes = plane_bytes(s) * (size_t)s.nz;
+  if ((!force && bytes < kSlabMinBytes) || s.kz < 1) return best;
+  const int min_slabs = force ? 2 : 4;
   int lens[64];
   const int cnt = fcamd_static_lengths(lens, 64);
   for (int i = 0; i < cnt && i < 64; ++i) {
-    const int P = lens[i], m = P - (s.kz - 1);
-    if (m < 8 || m < 2 * (s.kz - 1)) continue;
-    if ((s.nz + m - 1) / m < 4) continue;
-    if (P > p.P) {
-      p.P = P;
-      p.m = m;
-    }
+    const int m0 = lens[i] - (s.kz - 1);
+    if (m0 < 8 || m0 < 2 * (s.kz - 1)) continue;
+    const int n = (s.nz + m0 - 1) / m0;
+    if (n < min_slabs) continue;
+    // balanced slabs, all transformed at one length
+    SlabPlan c{};
+    c.nslabs = n;
+    c.m = (s.nz + n - 1) / n;
+    c.P = fast_length(c.m + s.kz - 1);
+    // the first window must not wrap onto itself
+    if (c.m + s.kz - 1 > s.nz) continue;
+    size_plan(s, c);
+    if (c.device_bytes() > budget) continue;
+    if (c.m > best.m) best = c;
   }
-  if (p.m > 0) {
-    // balanced slabs, all transformed at one compile-time length
-    p.nslabs = (s.nz + p.m - 1) / p.m;
-    p.m = (s.nz + p.nslabs - 1) / p.nslabs;
-    p.P = fast_length(p.m + s.kz - 1);
-  }
-  return p;
+  return best;
 }
 
 PaddedProblem slab_problem(const ConvShape& s, int z0, int m, int P) {
@@ -130,8 +157,7 @@ PaddedProblem slab_problem(const ConvShape& s, int z0, int m, int P) {
   v.m[2] = m;
   FC_REQUIRE(P >= m + s.kz - 1, kInternal, "slab transform too short");
   p.padded.nz = P;
-  // the window the slab needs; planes beyond it read zero (they may not
-  // have been uploaded yet)
+  // the window the slab needs; planes beyond it read zero
   v.hi[0] = s.nx;
   v.hi[1] = s.ny;
   v.hi[2] = m + s.kz - 1;
@@ -140,93 +166,150 @@ PaddedProblem slab_problem(const ConvShape& s, int z0, int m, int P) {
   return p;
 }
 
-size_t slab_workspace_bytes(const ConvShape& s, const SlabPlan& sp) {
-  size_t w = 0;
-  for (int i = 0; i < sp.nslabs; ++i) {
-    const int z0 = i * sp.m, m = std::min(sp.m, s.nz - z0);
-    w = std::max(w, workspace_layout(slab_problem(s, z0, m, sp.P).padded).total());
-  }
-  return w;
+PaddedProblem slab_window_problem(const ConvShape& s, int m, int P) {
+  const int lo = s.kz - 1 - s.cz, W = m + s.kz - 1;
+  PaddedProblem p = slab_problem(s, 0, m, P);
+  // padded position p reads buffer plane p - lo + off = p (p < W), and the
+  // cropped output plane p - lo goes to buffer plane p - lo
+  p.vp.n[2] = W;
+  p.vp.off[2] = lo;
+  return p;
 }
 
 void host_convolution_slabs(const ConvShape& s, const SlabPlan& sp, const float* im,
-                            const float* kernel, float* out, Op op, int dev, float* d_in,
-                            float* d_out, float* d_ker, void* ws, hipStream_t compute,
+                            const float* kernel, float* out, Op op, int dev, float* ring,
+                            float* wrap, float* d_ker, void* ws, hipStream_t compute,
                             void (*host_copy)(void*, const void*, size_t)) {
-  const size_t plane = sizeof(float) * (size_t)s.nx * s.ny;
+  const size_t plane = plane_bytes(s);
   const int cp = std::max<int>(1, (int)(kSlabChunk / plane));  // planes per chunk
   SlabStreams& ss = slab_streams(dev, plane * cp);
-  const int lo = s.kz - 1 - s.cz, nz = s.nz;
+  const int nz = s.nz, cz = s.cz, lo = s.kz - 1 - s.cz, halo = s.kz - 1;
+  const int NB = kRingBuffers;
+  const size_t stride = sp.ring_bytes / NB;  // bytes per ring buffer
+  FC_REQUIRE(stride >= ((size_t)sp.m + halo) * plane, kInternal, "ring buffer too small");
+  auto buf = [&](int k) { return reinterpret_cast<char*>(ring) + (size_t)(k % NB) * stride; };
 
-  // upload chunks: the tail planes [nz - lo, nz) first, then [0, nz - lo)
-  struct Range {
-    int z0, n;
+  struct Slab {
+    int z0, m;
+    int first_up = -1, last_up = -1;      // upload chunk range
+    int first_down = -1, last_down = -1;  // download chunk range
   };
-  std::vector<Range> up;
-  const int head = nz - lo;
-  for (int z = head; z < nz; z += cp) up.push_back({z, std::min(cp, nz - z)});
-  const size_t ntail = up.size();
-  for (int z = 0; z < head; z += cp) up.push_back({z, std::min(cp, head - z)});
-  // planes of [0, head) covered after upload chunk i (tail chunks cover none)
-  auto covered = [&](size_t i) { return i < ntail ? 0 : up[i].z0 + up[i].n; };
+  std::vector<Slab> slabs(sp.nslabs);
+  for (int k = 0; k < sp.nslabs; ++k) {
+    slabs[k].z0 = k * sp.m;
+    slabs[k].m = std::min(sp.m, nz - slabs[k].z0);
+    FC_REQUIRE(slabs[k].m >= 1, kInternal, "empty slab");
+  }
 
-  // download chunks, slab by slab
+  // upload chunks: host planes [src, src + n) into buffer planes [dst, dst + n)
+  struct Up {
+    int slab, src, dst, n;
+  };
+  std::vector<Up> up;
+  auto add_up = [&](int k, int src, int dst, int n) {
+    for (int i = 0; i < n; i += cp) {
+      if (slabs[k].first_up < 0) slabs[k].first_up = (int)up.size();
+      slabs[k].last_up = (int)up.size();
+      up.push_back({k, src + i, dst + i, std::min(cp, n - i)});
+    }
+  };
+  for (int k = 0; k < sp.nslabs; ++k) {
+    const Slab& sl = slabs[k];
+    if (k == 0) {
+      // the whole window: the tail planes [nz - lo, nz), then [0, m + cz)
+      add_up(0, nz - lo, 0, lo);
+      add_up(0, 0, lo, sl.m + cz);
+    } else {
+      // the m new planes (window positions halo..W-1, sources z0 + cz ..),
+      // those below nz from the host; the rest wrap to the saved planes
+      const int s0 = sl.z0 + cz, s1 = std::min(sl.z0 + sl.m + cz, nz);
+      if (s1 > s0) add_up(k, s0, halo, s1 - s0);
+    }
+  }
+  // download chunks: buffer planes [0, m) of each slab to host [z0, z0 + m)
   struct Down {
-    int slab, z0, n;
+    int slab, z, n;
   };
   std::vector<Down> down;
   for (int k = 0; k < sp.nslabs; ++k) {
-    const int z0 = k * sp.m, m = std::min(sp.m, nz - z0);
-    for (int z = z0; z < z0 + m; z += cp) down.push_back({k, z, std::min(cp, z0 + m - z)});
+    Slab& sl = slabs[k];
+    for (int i = 0; i < sl.m; i += cp) {
+      if (sl.first_down < 0) sl.first_down = (int)down.size();
+      sl.last_down = (int)down.size();
+      down.push_back({k, i, std::min(cp, sl.m - i)});
+    }
   }
 
   EventPool ev_up, ev_conv, ev_down;
   FC_HIP_CHECK(hipMemcpyAsync(d_ker, kernel, sizeof(float) * (size_t)s.kx * s.ky * s.kz,
                               hipMemcpyHostToDevice, ss.up));
+  const size_t up_total = up.size(), down_total = down.size();
   size_t up_next = 0, down_issue = 0, down_done = 0;
   int slab_next = 0, slab_waited_down = -1;
-  const size_t up_total = up.size(), down_total = down.size();
+  // downloads of slab k all enqueued (its last-chunk event recorded)
+  auto downs_issued = [&](int k) { return (int)down_issue > slabs[k].last_down; };
   while (down_done < down_total) {
     bool progress = false;
-    // 1. one upload chunk if its pinned buffer is free
+    // 1. one upload chunk, once its pinned buffer is free and its ring
+    //    buffer's previous slab no longer needs it
     if (up_next < up_total) {
-      const int b = (int)(up_next % kUpBufs);
-      const bool free_buf = up_next < (size_t)kUpBufs || done(ev_up.get(up_next - kUpBufs));
-      if (free_buf) {
-        const Range r = up[up_next];
-        host_copy(ss.ubuf[b], reinterpret_cast<const char*>(im) + plane * r.z0, plane * r.n);
-        FC_HIP_CHECK(hipMemcpyAsync(reinterpret_cast<char*>(d_in) + plane * r.z0, ss.ubuf[b],
-                                    plane * r.n, hipMemcpyHostToDevice, ss.up));
+      const Up& u = up[up_next];
+      const int k = u.slab;
+      const bool first = (int)up_next == slabs[k].first_up;
+      // buffer k mod NB last held slab k - NB (downloaded from it) and fed
+      // slab k - NB + 1 its halo
+      const bool ring_free = k < NB || (downs_issued(k - NB) && slab_next > k - NB + 1);
+      const bool pinned_free =
+          up_next < (size_t)kUpBufs || done(ev_up.get(up_next - kUpBufs));
+      if (ring_free && pinned_free) {
+        if (first && k >= NB) {
+          FC_HIP_CHECK(hipStreamWaitEvent(ss.up, ev_down.get(slabs[k - NB].last_down), 0));
+          FC_HIP_CHECK(hipStreamWaitEvent(ss.up, ev_conv.get(k - NB + 1), 0));
+        }
+        const int b = (int)(up_next % kUpBufs);
+        host_copy(ss.ubuf[b], reinterpret_cast<const char*>(im) + plane * u.src, plane * u.n);
+        FC_HIP_CHECK(hipMemcpyAsync(buf(k) + plane * u.dst, ss.ubuf[b], plane * u.n,
+                                    hipMemcpyHostToDevice, ss.up));
         FC_HIP_CHECK(hipEventRecord(ev_up.get(up_next), ss.up));
         ++up_next;
         progress = true;
       }
     }
-    // 2. every slab whose input planes have all been enqueued for upload
+    // 2. the next slab's convolution once its uploads are enqueued
     while (slab_next < sp.nslabs) {
-      const int z0 = slab_next * sp.m, m = std::min(sp.m, nz - z0);
-      // planes of [0, head) it reads (its window is z0 - lo .. z0 + m + cz - 1
-      // mod nz; the tail planes went first)
-      const int need = std::min(z0 + m + s.cz, head);
-      if (up_next < ntail || up_next == 0) break;
-      size_t last = 0;
-      bool ready = false;
-      for (size_t i = ntail; i < up_next; ++i)
-        if (covered(i) >= need) {
-          last = i;
-          ready = true;
-          break;
+      const int k = slab_next;
+      const Slab& sl = slabs[k];
+      if (sl.last_up >= 0 && (int)up_next <= sl.last_up) break;
+      if (k >= NB && !downs_issued(k - NB)) break;
+      if (sl.last_up >= 0) FC_HIP_CHECK(hipStreamWaitEvent(compute, ev_up.get(sl.last_up), 0));
+      if (k >= NB)  // the halo copy below overwrites planes slab k - NB is downloaded from
+        FC_HIP_CHECK(hipStreamWaitEvent(compute, ev_down.get(slabs[k - NB].last_down), 0));
+      if (k == 0 && cz > 0)  // keep the planes the last window wraps to
+        FC_HIP_CHECK(hipMemcpyAsync(wrap, buf(0) + plane * lo, plane * cz,
+                                    hipMemcpyDeviceToDevice, compute));
+      if (k > 0 && halo > 0) {
+        const Slab& pr = slabs[k - 1];
+        FC_HIP_CHECK(hipMemcpyAsync(buf(k), buf(k - 1) + plane * pr.m, plane * halo,
+                                    hipMemcpyDeviceToDevice, compute));
+      }
+      if (k > 0) {
+        // window sources at or past nz wrap to the saved planes [0, cz)
+        const int s_lo = std::max(sl.z0 + cz, nz), s_hi = sl.z0 + sl.m + cz;
+        if (s_hi > s_lo) {
+          const int p0 = s_lo - (sl.z0 - lo);
+          FC_HIP_CHECK(hipMemcpyAsync(buf(k) + plane * p0,
+                                      reinterpret_cast<const char*>(wrap) + plane * (s_lo - nz),
+                                      plane * (s_hi - s_lo), hipMemcpyDeviceToDevice, compute));
         }
-      if (!ready) break;
-      FC_HIP_CHECK(hipStreamWaitEvent(compute, ev_up.get(last), 0));
-      const PaddedProblem p = slab_problem(s, z0, m, sp.P);
-      enqueue_convolution_padded(p, d_out + (size_t)z0 * s.nx * s.ny, d_in, d_ker, op, ws,
-                                 compute);
-      FC_HIP_CHECK(hipEventRecord(ev_conv.get(slab_next), compute));
+      }
+      const PaddedProblem p = slab_window_problem(s, sl.m, sp.P);
+      float* window = reinterpret_cast<float*>(buf(k));
+      enqueue_convolution_padded(p, window, window, d_ker, op, ws, compute);
+      FC_HIP_CHECK(hipEventRecord(ev_conv.get(k), compute));
       ++slab_next;
       progress = true;
     }
-    // 3. issue downloads of computed slabs into free pinned buffers
+    // 3. downloads of computed slabs into free pinned buffers
     while (down_issue < down_total && down_issue - down_done < (size_t)kDownBufs &&
            down[down_issue].slab < slab_next) {
       const Down d = down[down_issue];
@@ -235,8 +318,8 @@ void host_convolution_slabs(const ConvShape& s, const SlabPlan& sp, const float*
         slab_waited_down = d.slab;
       }
       const int b = (int)(down_issue % kDownBufs);
-      FC_HIP_CHECK(hipMemcpyAsync(ss.dbuf[b], reinterpret_cast<const char*>(d_out) + plane * d.z0,
-                                  plane * d.n, hipMemcpyDeviceToHost, ss.down));
+      FC_HIP_CHECK(hipMemcpyAsync(ss.dbuf[b], buf(d.slab) + plane * d.z, plane * d.n,
+                                  hipMemcpyDeviceToHost, ss.down));
       FC_HIP_CHECK(hipEventRecord(ev_down.get(down_issue), ss.down));
       ++down_issue;
       progress = true;
@@ -244,8 +327,8 @@ void host_convolution_slabs(const ConvShape& s, const SlabPlan& sp, const float*
     // 4. copy out finished downloads (in order)
     while (down_done < down_issue && done(ev_down.get(down_done))) {
       const Down d = down[down_done];
-      host_copy(reinterpret_cast<char*>(out) + plane * d.z0, ss.dbuf[down_done % kDownBufs],
-                plane * d.n);
+      host_copy(reinterpret_cast<char*>(out) + plane * ((size_t)slabs[d.slab].z0 + d.z),
+                ss.dbuf[down_done % kDownBufs], plane * d.n);
       ++down_done;
       progress = true;
     }

@@ -73,9 +73,12 @@ FUNCTION_PREFIX long long int getMemDeviceCUDA(int devCUDA);
  * (major*10000000 + minor*100000 + patch). */
 FUNCTION_PREFIX int cuda_version();
 
-/* src/convolution3Dfft.h:75-86 (src/convolution3Dfft.cu:587-603): device
- * workspace in MiB this library needs for an R2C transform of `shape`
- * (len 1..3, last entry contiguous); -1 for an invalid request. */
+/* src/convolution3Dfft.h:75-86 (src/convolution3Dfft.cu:587-603, there
+ * cuFFT's work area): the device memory in MiB (rounded up) a host-pointer
+ * convolution of an image of `shape` (len 1..3, last entry contiguous)
+ * allocates on the current device for a one-voxel kernel, within the device
+ * budget (fcamd_host_device_bytes in fourierconvolution_amd.h gives the
+ * figure for any kernel); -1 for an invalid request. */
 FUNCTION_PREFIX int gpu_mem_needed_mb(int* shape, int len);
 
 #endif /* __CONVOLUTION_3D_FFT_H__ */

@@ -34,6 +34,16 @@ FCAMD_API void fcamd_free_result(float* p);
 FCAMD_API int fcamd_convolve3d_host(float* im, const int* imDim, const float* kernel,
                                     const int* kernelDim, int op, int devCUDA);
 
+/* Device memory, in bytes, that convolution3DfftCUDAInPlace /
+ * fcamd_convolve3d_host allocate on device devCUDA for these dims (the
+ * reference reports only cuFFT's work area, src/convolution3Dfft.cu:587-603).
+ * Host-pointer calls of 32 MiB or more, and any call whose single-shot
+ * buffers exceed the device budget, stream z-slabs through a ring of slab
+ * buffers (no whole volume on the device).  The budget is
+ * FCAMD_DEVICE_BUDGET_MB when set (read on every call; then no device is
+ * needed for this query), else the device's free memory.  -1 on error. */
+FCAMD_API long long fcamd_host_device_bytes(const int* imDim, const int* kernelDim, int devCUDA);
+
 /* Device-resident form of the hot path: every pointer is device memory on
  * the current device; d_out may alias d_image; `stream` is a hipStream_t
  * (NULL = the null stream).  d_workspace may be NULL (a per-device pooled

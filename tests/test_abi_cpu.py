@@ -100,9 +100,37 @@ def test_batch_workspace_lanes(fc, monkeypatch):
     assert lib.fcamd_batch_workspace_bytes(n, dims, kdims) == need(4)
 
 
-def test_gpu_mem_needed_mb(fc):
-    assert fc.gpu_mem_needed_mb([256, 512, 512]) == (256 * 512 * 272 * 8) >> 20
-    assert fc.lib().gpu_mem_needed_mb(None, 3) == -1
+def test_gpu_mem_needed_mb(fc, monkeypatch):
+    """gpu_mem_needed_mb and fcamd_host_device_bytes report what the
+    host-pointer path allocates: the single shot (image, workspace, kernel)
+    for small volumes, the z-slab ring (within the budget) for large ones.
+    With FCAMD_DEVICE_BUDGET_MB set no device is needed to answer."""
+    from fourierconvolutioncudalib_amd._lib import ints
+    lib = fc.lib()
+    monkeypatch.setenv("FCAMD_DEVICE_BUDGET_MB", "100000")
+    a256 = lambda b: (b + 255) // 256 * 256  # noqa: E731
+    # small volume: single shot
+    shape, kshape = (16, 24, 40), (3, 5, 7)
+    single = (a256(4 * 16 * 24 * 40) + fc.workspace_bytes(shape, kshape) + a256(4 * 3 * 5 * 7))
+    assert lib.fcamd_host_device_bytes(ints(shape[::-1]), ints(kshape[::-1]), 0) == single
+    one = a256(4 * 16 * 24 * 40) + fc.workspace_bytes(shape, (1, 1, 1)) + 256
+    assert fc.gpu_mem_needed_mb(list(shape)) == -(-one // (1 << 20))
+    # 512 x 512 x 256 streams through the slab ring: less than one volume
+    # plus its workspace, more than one slab window
+    big, k15 = (256, 512, 512), (15, 15, 15)
+    ring = lib.fcamd_host_device_bytes(ints(big[::-1]), ints(k15[::-1]), 0)
+    plane = 4 * 512 * 512
+    assert 16 * plane < ring < 4 * 256 * plane + fc.workspace_bytes(big, k15)
+    # a tighter budget gives smaller slabs, within it
+    monkeypatch.setenv("FCAMD_DEVICE_BUDGET_MB", "300")
+    tight = lib.fcamd_host_device_bytes(ints(big[::-1]), ints(k15[::-1]), 0)
+    assert tight <= 300 << 20 and tight < ring
+    assert fc.gpu_mem_needed_mb([256, 512, 512]) <= 300
+    # below the smallest slab (48-plane transforms need about 215 MiB here)
+    # nothing fits: the figure is the single shot's, which the call refuses
+    monkeypatch.setenv("FCAMD_DEVICE_BUDGET_MB", "200")
+    assert lib.fcamd_host_device_bytes(ints(big[::-1]), ints(k15[::-1]), 0) > 200 << 20
+    assert lib.gpu_mem_needed_mb(None, 3) == -1
     assert fc.gpu_mem_needed_mb([0, 4, 4]) == -1
 
 
