@@ -43,6 +43,7 @@ SIGNATURES = {
     "fcamd_workspace_bytes": (_ll, [_ip, _ip]),
     "fcamd_host_device_bytes": (_ll, [_ip, _ip, _i]),
     "fcamd_convolve3d_device": (_i, [_vp, _vp, _ip, _vp, _ip, _i, _vp, _ll, _vp]),
+    "fcamd_convolve3d_device_noalloc": (_i, [_vp, _vp, _ip, _vp, _ip, _i, _vp]),
     "fcamd_radix_plan": (_i, [_i, _ip, _i]),
     "fcamd_grad_kernel_workspace_bytes": (_ll, [_ip, _ip]),
     "fcamd_conv_grad_kernel_device": (_i, [_vp, _vp, _vp, _ip, _ip, _vp, _ll, _vp]),

@@ -17,7 +17,9 @@
 #include "conv_driver.hpp"
 #include "../../include/fourierconvolution_amd.h"
 
+#include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <map>
 #include <memory>
 #include <string>
@@ -30,18 +32,45 @@ namespace fcamd {
 // ---------------------------------------------------------------------------
 // plans and twiddle tables
 // ---------------------------------------------------------------------------
-std::vector<int> radix_plan(int L) {
-  std::vector<int> r;
-  int n = L;
-  while (n % 8 == 0) { r.push_back(8); n /= 8; }
-  if (n % 4 == 0) { r.push_back(4); n /= 4; }
-  if (n % 2 == 0) { r.push_back(2); n /= 2; }
+int radix_plan_into(int L, int* rad, int cap) {
+  int k = 0, n = L;
+  auto put = [&](int r) {
+    if (k < cap) rad[k] = r;
+    ++k;
+  };
+  while (n % 8 == 0) { put(8); n /= 8; }
+  if (n % 4 == 0) { put(4); n /= 4; }
+  if (n % 2 == 0) { put(2); n /= 2; }
   for (int p : {3, 5, 7})
-    while (n % p == 0) { r.push_back(p); n /= p; }
+    while (n % p == 0) { put(p); n /= p; }
   for (int p = 11; (long)p * p <= n; p += 2)
-    while (n % p == 0) { r.push_back(p); n /= p; }
-  if (n > 1) r.push_back(n);
-  return r;
+    while (n % p == 0) { put(p); n /= p; }
+  if (n > 1) put(n);
+  return k;
+}
+
+std::vector<int> radix_plan(int L) {
+  int rad[64];
+  const int k = radix_plan_into(L, rad, 64);
+  return std::vector<int>(rad, rad + (k < 64 ? k : 64));
+}
+
+int stage_table_segments(int L, TwSegment* seg, int* total) {
+  const StaticPlanData p = static_plan(L);
+  *total = 0;
+  if (p.n <= 0) return 0;
+  int n = 0;
+  for (int rev = 0; rev < 2; ++rev) {
+    int ns = 1;
+    for (int i = 0; i < p.n; ++i) {
+      const int r = rev ? p.r[p.n - 1 - i] : p.r[i];
+      if (n < kMaxTwSegments) seg[n] = TwSegment{*total, ns, r, even_up(r)};
+      ++n;
+      *total += ns * even_up(r);
+      ns *= r;
+    }
+  }
+  return n;
 }
 
 namespace {
@@ -70,27 +99,25 @@ float2* twiddle_table(int dev, int L) {
 // Per-stage tables of the compile-time plan of L, forward then reversed
 // stage order (layout in fft_static.hpp); empty when L has no static plan.
 std::vector<float2> static_stage_table(int L) {
-  const StaticPlanData p = static_plan(L);
-  std::vector<float2> out;
-  if (p.n <= 0) return out;
-  for (int rev = 0; rev < 2; ++rev) {
-    long ns = 1;
-    for (int i = 0; i < p.n; ++i) {
-      const int r = rev ? p.r[p.n - 1 - i] : p.r[i];
-      const int rp = even_up(r);
-      const long span = ns * r;
-      for (long k = 0; k < ns; ++k)
-        for (int j = 0; j < rp; ++j) {
-          if (j >= r) {
-            out.push_back(make_float2(1.f, 0.f));
-            continue;
-          }
-          const long m = ((long)j * k) % span;
-          const double a = -2.0 * M_PI * (double)m / (double)span;
-          out.push_back(make_float2((float)std::cos(a), (float)std::sin(a)));
+  TwSegment seg[kMaxTwSegments];
+  int total = 0;
+  const int n = stage_table_segments(L, seg, &total);
+  FC_REQUIRE(n <= kMaxTwSegments, kInternal, "too many twiddle stages");
+  std::vector<float2> out((size_t)total);
+  for (int i = 0; i < n; ++i) {
+    const TwSegment& g = seg[i];
+    const long span = (long)g.ns * g.r;
+    for (long k = 0; k < g.ns; ++k)
+      for (int j = 0; j < g.rp; ++j) {
+        float2& e = out[(size_t)g.offset + (size_t)k * g.rp + j];
+        if (j >= g.r) {
+          e = make_float2(1.f, 0.f);
+          continue;
         }
-      ns *= r;
-    }
+        const long m = ((long)j * k) % span;
+        const double a = -2.0 * M_PI * (double)m / (double)span;
+        e = make_float2((float)std::cos(a), (float)std::sin(a));
+      }
   }
   return out;
 }
@@ -128,7 +155,18 @@ std::once_flag g_attr_once;
 
 }  // namespace
 
+namespace {
+thread_local const FftDescSet* t_desc_override = nullptr;
+}  // namespace
+
+void set_fft_desc_override(const FftDescSet* set) { t_desc_override = set; }
+
 FftDesc fft_desc(int L, hipStream_t st) {
+  if (const FftDescSet* o = t_desc_override) {
+    for (int i = 0; i < o->n; ++i)
+      if (o->d[i].L == L) return o->d[i];
+    throw Error{kInternal, "transform length without prepared tables"};
+  }
   FftDesc d{};
   d.L = L;
   std::vector<int> r = radix_plan(L);

@@ -44,6 +44,29 @@ FftDesc fft_desc(int L, hipStream_t st = nullptr);
 
 // Radix plan used for length L.
 std::vector<int> radix_plan(int L);
+// The same plan written into rad[0..cap) without allocating; returns the
+// stage count (which may exceed cap).
+int radix_plan_into(int L, int* rad, int cap);
+
+// Layout of the per-stage twiddle table of the compile-time plan of L
+// (forward stage order, then reversed): one segment per stage, `ns` rows of
+// `rp` entries from element `offset` on, row k entry r = exp(-2 pi i
+// (r k mod ns R) / (ns R)) for r < R and 1 for R <= r < rp.  Returns the
+// segment count (0: L has no compile-time plan) and the total entries.
+struct TwSegment {
+  int offset, ns, r, rp;
+};
+constexpr int kMaxTwSegments = 32;
+int stage_table_segments(int L, TwSegment* seg, int* total);
+
+// While set on the calling thread, fft_desc() serves the lengths held here
+// (tables in caller-provided device memory) and creates no table; any other
+// length is an internal error.  nullptr clears it.
+struct FftDescSet {
+  int n = 0;
+  FftDesc d[3];
+};
+void set_fft_desc_override(const FftDescSet* set);
 
 enum class Op : int { kConvolve = 0, kCorrelate = 1 };
 

@@ -54,6 +54,23 @@ FCAMD_API int fcamd_convolve3d_device(float* d_out, const float* d_image, const 
                                       const float* d_kernel, const int* kernelDim, int op,
                                       void* d_workspace, long long workspace_bytes, void* stream);
 
+/* fcamd_convolve3d_device without allocation (no counterpart in the
+ * reference): for callers that cannot allocate device memory -- code under
+ * graph capture, tight serving loops, harnesses that forbid allocation.  The
+ * workspace and the twiddle tables live in one device array of the library,
+ * FCAMD_NOALLOC_MAX_WORKSPACE bytes of workspace plus 1 MiB of tables, that
+ * each device holds until the process ends; a call whose
+ * fcamd_workspace_bytes() exceeds FCAMD_NOALLOC_MAX_WORKSPACE fails with code
+ * 2 (unsupported size).  The call allocates nothing, so it may be recorded
+ * into a graph, also for lengths the process has not used before.  Every
+ * call on one device shares that array: calls that overlap on one device
+ * (other threads, other streams) race; order them.  Same results as
+ * fcamd_convolve3d_device; d_out may alias d_image. */
+#define FCAMD_NOALLOC_MAX_WORKSPACE (128LL << 20)
+FCAMD_API int fcamd_convolve3d_device_noalloc(float* d_out, const float* d_image,
+                                              const int* imDim, const float* d_kernel,
+                                              const int* kernelDim, int op, void* stream);
+
 /* Adjoint with respect to the kernel (no counterpart in the reference; the
  * gradient half of the differentiable form).  For out = conv(im, k) and an
  * upstream gradient g (image-sized), writes
