@@ -71,6 +71,31 @@ ConvShape shape_from(const int* imDim, const int* kernelDim) {
   return s;
 }
 
+// shape_from for the entry points that also take the long-axis path
+ConvShape shape_from_any(const int* imDim, const int* kernelDim) {
+  FC_REQUIRE(imDim != nullptr && kernelDim != nullptr, kInvalidArgument, "null dimension array");
+  ConvShape s{imDim[0], imDim[1], imDim[2], kernelDim[0], kernelDim[1], kernelDim[2],
+              kernelDim[0] / 2, kernelDim[1] / 2, kernelDim[2] / 2};
+  if (needs_long_path(s))
+    validate_long_shape(s);
+  else
+    validate_shape(s);
+  return s;
+}
+
+// device workspace of one device-resident convolution of `s`
+size_t conv_workspace_bytes(const ConvShape& s) {
+  return needs_long_path(s) ? long_workspace_bytes(s) : workspace_layout(s).total();
+}
+
+void enqueue_any(const ConvShape& s, float* d_out, const float* d_image, const float* d_kernel,
+                 Op op, void* ws, hipStream_t st) {
+  if (needs_long_path(s))
+    enqueue_convolution_long(s, d_out, d_image, d_kernel, op, ws, st);
+  else
+    enqueue_convolution(s, d_out, d_image, d_kernel, op, ws, st);
+}
+
 // the drop-in kernel centring without the kernel <= image requirement (the
 // boundary-mode paths validate the padded transform shape instead)
 ConvShape raw_shape(const int* imDim, const int* kernelDim) {
@@ -130,6 +155,12 @@ struct HostPlan {
 
 HostPlan host_plan(const ConvShape& s, size_t budget) {
   HostPlan h;
+  if (needs_long_path(s)) {  // long axes: one shot, no slabs
+    h.device_bytes = (sizeof(float) * (size_t)s.voxels() + 255) / 256 * 256 +
+                     long_workspace_bytes(s) +
+                     (sizeof(float) * (size_t)s.kx * s.ky * s.kz + 255) / 256 * 256;
+    return h;
+  }
   const size_t single = single_shot_device_bytes(s);
   const size_t nker = (sizeof(float) * (size_t)s.kx * s.ky * s.kz + 255) / 256 * 256;
   if (slabs_enabled()) {
@@ -173,14 +204,14 @@ void host_convolution(const ConvShape& s, const float* im, const float* kernel, 
              "the convolution needs " + std::to_string(hp.device_bytes >> 20) +
                  " MiB of device memory (device budget " + std::to_string(budget >> 20) +
                  " MiB) and no z-slab plan fits the budget");
-  const size_t nws = workspace_layout(s).total();
+  const size_t nws = conv_workspace_bytes(s);
   void* dimg = pool_reserve(reinterpret_cast<void**>(&pool.img), &pool.img_bytes, nimg);
   void* dker = pool_reserve(reinterpret_cast<void**>(&pool.ker), &pool.ker_bytes, nker);
   void* ws = pool_reserve(&pool.ws, &pool.ws_bytes, nws);
   staged_upload(dev, dimg, im, nimg, pool.stream);
   staged_upload(dev, dker, kernel, nker, pool.stream);
-  enqueue_convolution(s, static_cast<float*>(dimg), static_cast<float*>(dimg),
-                      static_cast<float*>(dker), op, ws, pool.stream);
+  enqueue_any(s, static_cast<float*>(dimg), static_cast<float*>(dimg), static_cast<float*>(dker),
+              op, ws, pool.stream);
   staged_download(dev, out, dimg, nimg, pool.stream);
 }
 
@@ -228,7 +259,7 @@ extern "C" {
 void convolution3DfftCUDAInPlace(imageType* im, int* imDim, imageType* kernel, int* kernelDim,
                                  int devCUDA) {
   (void)guarded([&] {
-    ConvShape s = shape_from(imDim, kernelDim);
+    ConvShape s = shape_from_any(imDim, kernelDim);
     host_convolution(s, im, kernel, im, Op::kConvolve, devCUDA);
   });
 }
@@ -246,7 +277,7 @@ imageType* convolution3DfftCUDA(imageType* im, int* imDim, imageType* kernel, in
                "null dimension array");
     const int d[3] = {imDim[2], imDim[1], imDim[0]};
     const int k[3] = {kernelDim[2], kernelDim[1], kernelDim[0]};
-    ConvShape s = shape_from(d, k);
+    ConvShape s = shape_from_any(d, k);
     result = new float[(size_t)s.voxels()];
     host_convolution(s, im, kernel, result, Op::kConvolve, devCUDA);
   });
@@ -271,7 +302,10 @@ imageType* convolution3DfftCUDA_test(imageType* im, int* imDim, imageType* kerne
     s.ky = s.ny;
     s.kz = s.nz;
     s.cx = s.cy = s.cz = 0;
-    validate_shape(s);
+    if (needs_long_path(s))
+      validate_long_shape(s);
+    else
+      validate_shape(s);
     result = new float[(size_t)s.voxels()];
     host_convolution(s, im, kernel, result, Op::kConvolve, devCUDA);
   });
@@ -304,7 +338,7 @@ int gpu_mem_needed_mb(int* shape, int len) {
     int dev = 0;
     size_t env = 0;
     if (!env_budget(&env)) FC_HIP_CHECK(hipGetDevice(&dev));
-    b = host_device_bytes(shape_from(d, k), dev);
+    b = host_device_bytes(shape_from_any(d, k), dev);
   });
   if (rc != kOk || b < 0) return -1;
   return (int)((b + (1 << 20) - 1) >> 20);
@@ -375,20 +409,20 @@ void fcamd_free_result(float* p) { delete[] p; }
 int fcamd_convolve3d_host(float* im, const int* imDim, const float* kernel, const int* kernelDim,
                           int op, int devCUDA) {
   return guarded([&] {
-    ConvShape s = shape_from(imDim, kernelDim);
+    ConvShape s = shape_from_any(imDim, kernelDim);
     host_convolution(s, im, kernel, im, op_from(op), devCUDA);
   });
 }
 
 long long fcamd_host_device_bytes(const int* imDim, const int* kernelDim, int devCUDA) {
   long long r = -1;
-  (void)guarded([&] { r = host_device_bytes(shape_from(imDim, kernelDim), devCUDA); });
+  (void)guarded([&] { r = host_device_bytes(shape_from_any(imDim, kernelDim), devCUDA); });
   return r;
 }
 
 long long fcamd_workspace_bytes(const int* imDim, const int* kernelDim) {
   long long r = -1;
-  (void)guarded([&] { r = (long long)workspace_layout(shape_from(imDim, kernelDim)).total(); });
+  (void)guarded([&] { r = (long long)conv_workspace_bytes(shape_from_any(imDim, kernelDim)); });
   return r;
 }
 
@@ -397,20 +431,20 @@ int fcamd_convolve3d_device(float* d_out, const float* d_image, const int* imDim
                             long long workspace_bytes, void* stream) {
   return guarded([&] {
     FC_REQUIRE(d_out && d_image && d_kernel, kInvalidArgument, "null device pointer");
-    ConvShape s = shape_from(imDim, kernelDim);
+    ConvShape s = shape_from_any(imDim, kernelDim);
     const Op o = op_from(op);
     hipStream_t st = static_cast<hipStream_t>(stream);
-    const size_t need = workspace_layout(s).total();
+    const size_t need = conv_workspace_bytes(s);
     if (d_workspace) {
       FC_REQUIRE(workspace_bytes >= (long long)need, kInvalidArgument, "workspace too small");
-      enqueue_convolution(s, d_out, d_image, d_kernel, o, d_workspace, st);
+      enqueue_any(s, d_out, d_image, d_kernel, o, d_workspace, st);
     } else {
       int dev = 0;
       FC_HIP_CHECK(hipGetDevice(&dev));
       DevicePool& pool = device_pool(dev);
       std::lock_guard<std::mutex> lk(pool.mu);
       void* ws = pool_reserve(&pool.ws, &pool.ws_bytes, need);
-      enqueue_convolution(s, d_out, d_image, d_kernel, o, ws, st);
+      enqueue_any(s, d_out, d_image, d_kernel, o, ws, st);
       // the pooled workspace is reused by the next call: finish before unlocking
       FC_HIP_CHECK(hipStreamSynchronize(st));
     }

@@ -25,6 +25,7 @@
 #include <string>
 
 #include "fft_static.hpp"
+#include "kernels_long.hpp"
 #include "static_kernels.hpp"
 
 namespace fcamd {
@@ -158,6 +159,24 @@ std::once_flag g_attr_once;
 namespace {
 thread_local const FftDescSet* t_desc_override = nullptr;
 }  // namespace
+
+const float2* long_twiddle_table(int L, hipStream_t st) {
+  const int dev = current_device();
+  bool have = false;
+  {
+    std::lock_guard<std::mutex> lk(g_tw_mu);
+    have = g_tw.count(std::make_pair(dev, L)) != 0;
+  }
+  if (!have && st != nullptr) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    FC_HIP_CHECK(hipStreamIsCapturing(st, &cs));
+    FC_REQUIRE(cs == hipStreamCaptureStatusNone, kInvalidArgument,
+               "transform length " + std::to_string(L) +
+                   " is used for the first time inside a stream capture; run one call of "
+                   "this shape before capturing");
+  }
+  return twiddle_table(dev, L);
+}
 
 void set_fft_desc_override(const FftDescSet* set) { t_desc_override = set; }
 
@@ -321,6 +340,7 @@ void driver_init() {
   std::call_once(g_attr_once, [] {
     generic_kernels_allow_large_lds();
     static_kernels_allow_large_lds();
+    long_kernels_allow_large_lds();
   });
 }
 
@@ -423,7 +443,7 @@ void forward_xy(const ConvShape& s, const float* src, float2* S, const float* ke
 }
 
 void z_pass(const ConvShape& s, float2* S, const float2* operand, bool dense, bool conj,
-            hipStream_t st) {
+            hipStream_t st, int out_planes, int out_shift) {
   ensure_kernel_attributes();
   const long nxp = padded_bins(s.nx);
   PassScope ps(kPassZFused, st);
@@ -441,6 +461,10 @@ void z_pass(const ConvShape& s, float2* S, const float2* operand, bool dense, bo
   a.center = dense ? 0 : s.cz;
   a.conj = conj ? 1 : 0;
   a.scale = (float)(1.0 / (double)s.voxels());
+  // 0 keeps the plain in-place store; a shift always takes the rotated one
+  a.out_planes = (out_planes < s.nz || out_shift != 0) ? std::min(out_planes, s.nz) : 0;
+  a.out_shift = out_shift;
+  FC_REQUIRE(a.out_shift >= 0 && a.out_shift < s.nz, kInternal, "z-pass output shift");
   if (!static_zfused(s, a, st)) launch_gk_zfused(a, s.ny, st);
 }
 
@@ -565,7 +589,7 @@ GradWorkspaceLayout grad_workspace_layout(const ConvShape& s) {
   const size_t nxp = (size_t)padded_bins(s.nx);
   GradWorkspaceLayout w;
   w.spectrum = align256(sizeof(float2) * nxp * s.ny * s.nz);
-  w.corr = align256(sizeof(float) * (size_t)s.voxels());
+  w.corr = align256(sizeof(float) * (size_t)s.nx * s.ny * s.kz);  // the kz support planes
   return w;
 }
 
@@ -577,24 +601,27 @@ void enqueue_grad_kernel(const ConvShape& s, float* d_dkernel, const float* d_gr
   float2* I = reinterpret_cast<float2*>(base + wl.spectrum);
   float* U = reinterpret_cast<float*>(base + 2 * wl.spectrum);
   // u = IFFT(FFT(g) * conj(FFT(im))) / N, the circular cross-correlation
-  // u[d] = sum_i g[i] im[i - d]; dk[j] = u[(j - c) mod n]
-  forward_xy(s, d_grad_out, G, nullptr, nullptr, nullptr, st);
-  forward_xy(s, d_image, I, nullptr, nullptr, nullptr, st);
-  z_pass(s, G, I, true, true, st);
-  // The gather reads u only on the z-planes (j - cz) mod nz, j < kz: the
-  // runs [0, kz - cz) and [nz - cz, nz), disjoint since kz <= nz.  The y and
-  // x inverses act within a plane, so each run is inverted as a volume of
-  // its own planes; the other planes of U are never written or read.
-  const long plane_bins = (long)padded_bins(s.nx) * s.ny;
-  const long plane_vox = (long)s.nx * s.ny;
-  const int runs[2][2] = {{0, s.kz - s.cz}, {s.nz - s.cz, s.nz}};
-  for (const auto& r : runs) {
-    if (r[1] <= r[0]) continue;
-    ConvShape sub = s;
-    sub.nz = r[1] - r[0];
-    inverse_yx(sub, G + r[0] * plane_bins, U + r[0] * plane_vox, st);
-  }
-  launch_gather_kernel_support(s, U, d_dkernel, st);
+  // u[d] = sum_i g[i] im[i - d]; dk[j] = u[(j - c) mod n].  The image goes
+  // through the x and y passes as their second operand (the slot the
+  // kernel's rows and planes use, here with the image's extents and no
+  // shift), so both volumes share each launch.
+  ConvShape both = s;
+  both.kx = s.nx;
+  both.ky = s.ny;
+  both.kz = s.nz;
+  both.cx = both.cy = both.cz = 0;
+  forward_xy(both, d_grad_out, G, d_image, I, I, st);
+  // The gather reads u only on the z-planes (j - cz) mod nz, j < kz.  The z
+  // pass writes just those, rotated by cz so plane (j - cz) mod nz lands on
+  // plane j: the support is the first kz planes, inverted (y and x act
+  // within a plane) as a volume of its own; U holds kz planes.
+  z_pass(s, G, I, true, true, st, s.kz, s.cz);
+  ConvShape sub = s;
+  sub.nz = s.kz;
+  inverse_yx(sub, G, U, st);
+  ConvShape g = sub;
+  g.cz = 0;
+  launch_gather_kernel_support(g, U, d_dkernel, st);
   FC_HIP_CHECK(hipGetLastError());
 }
 

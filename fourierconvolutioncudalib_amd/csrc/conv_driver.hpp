@@ -112,8 +112,10 @@ void pass_y_forward(const ConvShape& s, float2* S, const float2* Kx, float2* Kxy
 void pass_y_inverse(const ConvShape& s, float2* S, hipStream_t stream);
 void pass_x_inverse(const ConvShape& s, const float2* S, float* dst, hipStream_t stream,
                     const VPad* vp = nullptr);
+// out_planes / out_shift: write only result planes [0, out_planes), result
+// plane p going to (p + out_shift) mod nz (out_planes >= nz: all, in place)
 void z_pass(const ConvShape& s, float2* S, const float2* operand, bool dense, bool conj,
-            hipStream_t stream);
+            hipStream_t stream, int out_planes = 1 << 30, int out_shift = 0);
 void inverse_yx(const ConvShape& s, float2* S, float* dst, hipStream_t stream,
                 const VPad* vp = nullptr);
 
@@ -131,11 +133,26 @@ int fast_length(int n);
 void enqueue_convolution_padded(const PaddedProblem& p, float* d_out, const float* d_image,
                                 const float* d_kernel, Op op, void* ws, hipStream_t stream);
 
+// Shapes with an axis longer than one transform tile (x > 10239, y > 10240,
+// z > 6826) take the unfused long-axis path (long_path.cpp): four-step FFTs
+// through HBM for such axes, the kernel placed on the image grid.
+bool needs_long_path(const ConvShape& s);
+void validate_long_shape(const ConvShape& s);
+size_t long_workspace_bytes(const ConvShape& s);
+void enqueue_convolution_long(const ConvShape& s, float* d_out, const float* d_image,
+                              const float* d_kernel, Op op, void* ws, hipStream_t stream);
+// exp(-2 pi i j / L), j < L, on the current device (created once; a first
+// use while `stream` captures is refused like fft_desc's)
+const float2* long_twiddle_table(int L, hipStream_t stream);
+// dst <- scale * dst * src (conj(src) when conj != 0) over n complex values
+void launch_modulate(float2* dst, const float2* src, long n, float scale, int conj,
+                     hipStream_t stream);
+
 // Gradient of sum(g * conv(im, k)) with respect to the kernel taps:
 // dk[j] = sum_i g[i] im[(i - (j - c)) mod n] on the kernel support.
 struct GradWorkspaceLayout {
   size_t spectrum;  // two spectra of the image's size
-  size_t corr;      // image-sized float correlation volume
+  size_t corr;      // the correlation on the kz planes the gather reads
   size_t total() const { return 2 * spectrum + corr; }
 };
 GradWorkspaceLayout grad_workspace_layout(const ConvShape& s);

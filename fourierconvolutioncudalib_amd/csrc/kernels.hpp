@@ -132,6 +132,12 @@ struct ZArgs {
   int conj;             // use conj(FFTz(k)) (adjoint / correlation)
   float scale;          // 1 / (nx*ny*nz)
   long ny;              // tile rows
+  // 0: the result overwrites the column in place.  > 0: only planes
+  // [0, out_planes) of the result are written, result position p going to
+  // plane (p + out_shift) mod nz (the PSF gradient keeps just the planes its
+  // gather reads, in one contiguous run)
+  int out_planes;
+  int out_shift;
 };
 
 // Column tiles of TX consecutive bins over `nouter` rows.  When the bins end

@@ -215,7 +215,13 @@ gk_zfused(ZArgs a) {
   float2* R = gfft<+1, true>(A, f1, a.fft, g);
   for (int w = threadIdx.x; w < L * TX; w += blockDim.x) {
     const int c = w % TX, pos = w / TX;
-    if (x0 + c < a.nbins) a.s[y * a.s_row + (long)pos * a.s_plane + x0 + c] = R[pos * TX + c];
+    int q = pos;
+    if (a.out_planes > 0) {  // rotated subset (see ZArgs)
+      q = pos + a.out_shift;
+      if (q >= L) q -= L;
+      if (q >= a.out_planes) continue;
+    }
+    if (x0 + c < a.nbins) a.s[y * a.s_row + (long)q * a.s_plane + x0 + c] = R[pos * TX + c];
   }
 }
 

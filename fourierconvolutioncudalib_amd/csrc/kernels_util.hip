@@ -99,6 +99,15 @@ void launch_rl_update(float* psi, const float* acc, float inv_views, long n, hip
 #include "conv_driver.hpp"
 
 namespace fcamd {
+void launch_modulate(float2* dst, const float2* src, long n, float scale, int conj,
+                     hipStream_t st) {
+  long blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  if (blocks < 1) return;
+  hipLaunchKernelGGL(modulate_and_normalize, dim3((unsigned)blocks), dim3(256), 0, st, dst, src, n,
+                     scale, conj);
+}
+
 void launch_gather_kernel_support(const ConvShape& s, const float* u, float* dk, hipStream_t st) {
   const long K = (long)s.kx * s.ky * s.kz;
   long blocks = (K + 255) / 256;

@@ -810,9 +810,21 @@ struct ZTile {
       v[slot] = val;
     });
   }
+  // SUBSET: honour ZArgs::out_planes (compiled only where a dense operand,
+  // the PSF gradient's, can reach the kernel)
+  template <bool SUBSET>
   __device__ void store(const ZArgs& a, int t, const float2 (&v)[MAXE]) const {
     if (!ok) return;
     const long sp = a.s_plane;
+    if (SUBSET && a.out_planes > 0) {  // rotated subset (see ZArgs)
+      const int sh = a.out_shift, np = a.out_planes;
+      sfft_for_outputs<L, true>(t, [&](int slot, int pos) {
+        int q = pos + sh;
+        if (q >= L) q -= L;
+        if (q < np) col[(long)q * sp] = v[slot];
+      });
+      return;
+    }
     float2* ct = col + (long)t * sp;
     sfft_for_outputs<L, true>(t, [&](int slot, int pos) { ct[(long)(pos - t) * sp] = v[slot]; });
   }
@@ -912,7 +924,7 @@ __device__ __forceinline__ void zfused_sparse_body(const ZArgs& a, const ZTile<L
   sfft_butterflies<L, false, -1, 1>(w, t);
   zproduct<L>(v, w, t, a.conj != 0, a.scale);
   sfft_run<L, true, +1>(v, t, lds, addr, a.fft.stw);
-  tile.store(a, t, v);
+  tile.template store<false>(a, t, v);
 }
 
 // whether the two-stage tap path applies to this launch
@@ -950,7 +962,7 @@ __device__ __forceinline__ void zfused_fft_body(const ZArgs& a, long tile_index,
   sfft_run<L, false, -1>(w, t, lds, addr, a.fft.stw);
   zproduct<L>(v, w, t, a.conj != 0, a.scale);
   sfft_run<L, true, +1>(v, t, lds, addr, a.fft.stw);
-  tile.store(a, t, v);
+  tile.template store<PATH == 0>(a, t, v);
 }
 
 template <int L, int TX, int PATH>
@@ -985,7 +997,7 @@ sk_zfused_dft(ZArgs a) {
   kernel_spectrum_direct<L>(w, t, tile.ok, tile.kcol, a.k_plane, a.nvalid, a.center, a.fft.tw);
   zproduct<L>(v, w, t, a.conj != 0, a.scale);
   sfft_run<L, true, +1>(v, t, lds, addr, a.fft.stw);
-  tile.store(a, t, v);
+  tile.template store<false>(a, t, v);
 }
 
 // ---------------------------------------------------------------------------

@@ -1,9 +1,11 @@
-"""Transform lengths at the documented single-tile limits of the
-runtime-length kernels (x <= 10239, y <= 10240, z <= 6826: one row, column or
-z column of the transform held in 160 KiB of LDS), against the FP64 oracle,
-and one past each limit rejected with the unsupported-size status before any
-device work, leaving the image untouched; zero extents are rejected as
-invalid arguments.
+"""Transform lengths at the single-tile limits of the fused passes (x <= 10239,
+y <= 10240, z <= 6826: one row, column or z column of the transform held in
+160 KiB of LDS), and past them, where the long-axis path takes over (four-step
+FFTs through HBM for an axis beyond one tile, the reference's unfused
+structure otherwise: src/convolution3Dfft.cu:519-547), against the FP64
+oracle.  A length past one tile with no split into two tiles (a prime above
+10240) is rejected with the unsupported-size status before any device work,
+leaving the image untouched; zero extents are rejected as invalid arguments.
 
 The exact limits 10239 = 3 * 3413 and 6826 = 2 * 3413 run a direct 3413-point
 DFT stage; 10206 = 2 * 3^6 * 7 and 10240 = 2^11 * 5 run butterflies only.
@@ -46,15 +48,45 @@ def test_maximum_extents(fc, oracle, shape, kshape):
     assert oracle.rel_l2(out, ref) < TOL
 
 
-@pytest.mark.parametrize("shape", [(2, 3, 10240), (2, 10241, 3), (6827, 2, 3)],
+# past the single-tile limits: the long-axis path
+LONG_CASES = [((2, 3, 10240), (1, 3, 7)),     # x: 80 x 128 four-step
+              ((2, 10241, 3), (1, 9, 3)),     # y: 77 x 133 four-step
+              ((6827, 2, 3), (5, 1, 3)),      # z: one tile of the unfused path (prime)
+              ((2, 3, 20000), (1, 3, 9)),     # x: 125 x 160
+              ((8000, 2, 3), (7, 1, 3)),      # z: 80 x 100
+              ((3, 12000, 2), (1, 15, 1)),    # y: 100 x 120
+              ((7000, 1, 10500), (9, 1, 11))]  # x and z past their limits together
+
+
+@pytest.mark.parametrize("shape,kshape", LONG_CASES, ids=lambda v: "x".join(map(str, v)))
+def test_past_single_tile_limits(fc, oracle, shape, kshape):
+    import torch
+    from fourierconvolutioncudalib_amd import synth
+    print("long case", shape, kshape, flush=True)
+    stack = synth.uniform(shape, 4300 + sum(shape))
+    kernel = synth.random_psf(kshape, 4400 + sum(kshape))
+    ref = oracle.numpy_fft_conv(stack, kernel)
+    out = _run(fc, stack, kernel)
+    assert oracle.rel_l2(out, ref) < TOL
+    # device entry point with a caller workspace, and the correlation
+    ti = torch.from_numpy(stack).cuda()
+    tk = torch.from_numpy(kernel).cuda()
+    ws = torch.empty(fc.workspace_bytes(shape, kshape), dtype=torch.uint8, device="cuda")
+    got = fc.convolve3d(ti, tk, op=fc.CORRELATE, workspace=ws).cpu().numpy()
+    refc = oracle.numpy_fft_conv(stack, kernel, conj_kernel=True)
+    assert oracle.rel_l2(got, refc) < TOL
+
+
+@pytest.mark.parametrize("shape", [(2, 3, 10243), (2, 10247, 3), (10253, 2, 3)],
                          ids=lambda v: "x".join(map(str, v)))
-def test_past_limits_rejected(fc, shape):
+def test_prime_past_one_tile_rejected(fc, shape):
     im = np.ones(shape, np.float32)
     k = np.ones((1, 1, 1), np.float32)
     with pytest.raises(fc.FCError) as err:
         fc.convolution3DfftCUDAInPlace(im.reshape(-1), list(shape[::-1]), k.reshape(-1),
                                        [1, 1, 1], 0)
     assert err.value.code == UNSUPPORTED_SIZE
+    assert "no split" in str(err.value)
     assert np.all(im == 1)
 
 
