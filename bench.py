@@ -328,6 +328,44 @@ def boundary_comparison(fc, torch, img_np, psf_np, checks, reps=10):
     return out
 
 
+# the padded shapes of the reference's numerical-stability tests
+# (tests/test_gpu_numerical_stability.cpp:40-96, :330-382): every extent
+# carries a large prime factor (79, 109, 11 and 13, 173)
+PRIME_SHAPES = [((218, 158, 158), (91, 31, 31)), ((346, 286, 286), (91, 31, 31))]
+
+
+def prime_shapes_timing(fc, torch, checks, reps=10):
+    """The reference's prime-padded test shapes, device-resident, seeded
+    uniform images with Gaussian PSFs: time per convolution and roofline
+    fraction (these lengths run the runtime-length kernels, prime factors as
+    direct DFT stages)."""
+    from fourierconvolutioncudalib_amd import synth
+    out = []
+    for shape, kshape in PRIME_SHAPES:
+        img_np = synth.uniform(shape, 1500 + shape[0])
+        psf_np = synth.gaussian_psf(kshape, (2.0, 2.0, 6.0))
+        img = torch.from_numpy(img_np).cuda()
+        psf = torch.from_numpy(psf_np).cuda()
+        res = torch.empty_like(img)
+        ws = torch.empty(fc.workspace_bytes(shape, kshape), dtype=torch.uint8, device="cuda")
+        stream = torch.cuda.current_stream()
+        ms = timed_calls(torch, stream, lambda: fc.convolve3d(img, psf, out=res, workspace=ws,
+                                                                stream=stream), reps)
+        got = res.cpu().numpy()
+        name = "x".join(map(str, shape[::-1])) + "_k" + "x".join(map(str, kshape[::-1]))
+        checks.add(f"prime shape {name}", lambda O, got=got, a=img_np, k=psf_np:
+                   (got, O.numpy_fft_conv(a, k)))
+        nz, ny, nx = shape
+        kz, ky, kx = kshape
+        _, cb = pass_bytes(nx, ny, nz, kx, ky, kz)
+        out.append({"workload": name, "radix_plans_xyz": [fc.radix_plan(nx), fc.radix_plan(ny),
+                                                          fc.radix_plan(nz)],
+                    "ms_per_conv": ms, "voxels_per_s": nx * ny * nz / (ms * 1e-3),
+                    "conv_roofline_frac": cb / (ms * 1e-3) / HBM_PEAK})
+        log(f"prime shape {name}: {ms:.3f} ms")
+    return out
+
+
 def load_traffic(workload):
     path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
     if not os.path.exists(path):
@@ -429,7 +467,7 @@ def main():
                     help="skip the CPU leg (CPU baseline and the FP64 parity checks)")
     ap.add_argument("--no-batched", action="store_true", help="time the metric workload only")
     ap.add_argument("--no-autotune", action="store_true")
-    ap.add_argument("--only", choices=["batched", "rl", "adjoint", "configs"],
+    ap.add_argument("--only", choices=["batched", "rl", "adjoint", "configs", "primes"],
                     help="run one side measurement alone and print its JSON (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -447,6 +485,7 @@ def main():
             "adjoint": lambda: adjoint_timing(fc, torch, ck),
             "configs": lambda: [config_timing(fc, torch, synth.CFG1, 50, ck, jobs, None, True),
                                 config_timing(fc, torch, synth.CFG2, 20, ck, jobs, 16, False)],
+            "primes": lambda: prime_shapes_timing(fc, torch, ck),
         }
         print(json.dumps({args.only: sides[args.only]()}))
         return
@@ -627,6 +666,7 @@ def main():
         result["configs"] = [
             config_timing(fc, torch, synth.CFG1, 50, checks, cpu_jobs, None, True),
             config_timing(fc, torch, synth.CFG2, 20, checks, cpu_jobs, 16, False)]
+        result["reference_prime_shapes"] = prime_shapes_timing(fc, torch, checks)
     del img, psf, out, ws
     torch.cuda.empty_cache()
 
