@@ -19,6 +19,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <complex>
 #include <cstdlib>
 #include <map>
 #include <memory>
@@ -140,12 +141,6 @@ float2* stage_table(int dev, int L) {
   return d;
 }
 
-bool tables_cached(int dev, int L) {
-  std::lock_guard<std::mutex> lk(g_tw_mu);
-  const auto key = std::make_pair(dev, L);
-  return g_tw.count(key) != 0 && g_stw.count(key) != 0;
-}
-
 int current_device() {
   int dev = 0;
   FC_HIP_CHECK(hipGetDevice(&dev));
@@ -180,20 +175,101 @@ const float2* long_twiddle_table(int L, hipStream_t st) {
 
 void set_fft_desc_override(const FftDescSet* set) { t_desc_override = set; }
 
+namespace {
+
+// Bluestein's chirp-z replaces plans with a prime factor above this (each
+// output of a direct p-point stage costs p complex multiply-adds)
+constexpr int kBluesteinPrime = 31;
+// ... when the convolution length fits the smallest tile of the passes
+// (the runtime-length z pass: three buffers in 160 KiB)
+constexpr int kBluesteinMaxM = 6826;
+
+int largest_prime_factor(int n) {
+  int best = 1;
+  for (int p = 2; (long)p * p <= n; ++p)
+    while (n % p == 0) {
+      best = p;
+      n /= p;
+    }
+  return n > 1 ? n : best;
+}
+
+int smooth_at_least(int n) {  // smallest 2^a 3^b 5^c 7^d >= n
+  for (int m = n;; ++m) {
+    int r = m;
+    for (int f : {2, 3, 5, 7})
+      while (r % f == 0) r /= f;
+    if (r == 1) return m;
+  }
+}
+
+// FP64 forward DFT of a 7-smooth length, recursive decimation in time (host)
+void host_dft64(std::vector<std::complex<double>>& a) {
+  const size_t n = a.size();
+  if (n <= 1) return;
+  size_t p = 2;
+  while (n % p) ++p;
+  const size_t m = n / p;
+  std::vector<std::vector<std::complex<double>>> sub(p, std::vector<std::complex<double>>(m));
+  for (size_t j = 0; j < m; ++j)
+    for (size_t r = 0; r < p; ++r) sub[r][j] = a[j * p + r];
+  for (auto& v : sub) host_dft64(v);
+  for (size_t k = 0; k < n; ++k) {
+    std::complex<double> acc = 0.0;
+    for (size_t r = 0; r < p; ++r) {
+      const double ang = -2.0 * M_PI * (double)((r * k) % n) / (double)n;
+      acc += sub[r][k % m] * std::complex<double>(std::cos(ang), std::sin(ang));
+    }
+    a[k] = acc;
+  }
+}
+
+// chirp[n] = exp(-i pi n^2 / L) (n < L) and bfft = DFT_M(b) / M with
+// b[m] = conj(chirp[|m|]) for |m| < L placed at m mod M, zero elsewhere
+void bluestein_tables(int L, int M, std::vector<float2>* chirp, std::vector<float2>* bfft) {
+  std::vector<std::complex<double>> w(L), b(M, 0.0);
+  for (int n = 0; n < L; ++n) {
+    const long long q = ((long long)n * n) % (2LL * L);  // exact angle
+    const double ang = -M_PI * (double)q / (double)L;
+    w[n] = std::complex<double>(std::cos(ang), std::sin(ang));
+  }
+  for (int n = 0; n < L; ++n) {
+    b[n] = std::conj(w[n]);
+    if (n > 0) b[M - n] = std::conj(w[n]);
+  }
+  host_dft64(b);
+  chirp->resize(L);
+  bfft->resize(M);
+  for (int n = 0; n < L; ++n) (*chirp)[n] = make_float2((float)w[n].real(), (float)w[n].imag());
+  for (int k = 0; k < M; ++k)
+    (*bfft)[k] = make_float2((float)(b[k].real() / M), (float)(b[k].imag() / M));
+}
+
+float2* upload_table(const std::vector<float2>& h) {
+  float2* d = nullptr;
+  FC_HIP_CHECK(hipMalloc(&d, sizeof(float2) * h.size()));
+  FC_HIP_CHECK(hipMemcpy(d, h.data(), sizeof(float2) * h.size(), hipMemcpyHostToDevice));
+  return d;
+}
+
+std::map<std::pair<int, int>, FftDesc> g_desc;  // (device, L) -> descriptor
+
+}  // namespace
+
 FftDesc fft_desc(int L, hipStream_t st) {
   if (const FftDescSet* o = t_desc_override) {
     for (int i = 0; i < o->n; ++i)
       if (o->d[i].L == L) return o->d[i];
     throw Error{kInternal, "transform length without prepared tables"};
   }
-  FftDesc d{};
-  d.L = L;
-  std::vector<int> r = radix_plan(L);
-  FC_REQUIRE((int)r.size() <= kMaxStages, kUnsupportedSize, "FFT plan too deep");
-  d.nst = (int)r.size();
-  for (int i = 0; i < d.nst; ++i) d.rad[i] = r[i];
   const int dev = current_device();
-  if (!tables_cached(dev, L) && st != nullptr) {
+  const auto key = std::make_pair(dev, L);
+  {
+    std::lock_guard<std::mutex> lk(g_tw_mu);
+    auto it = g_desc.find(key);
+    if (it != g_desc.end()) return it->second;
+  }
+  if (st != nullptr) {
     // creating the tables allocates and copies synchronously, which a
     // stream capture cannot record: refuse instead of breaking the capture
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
@@ -203,9 +279,33 @@ FftDesc fft_desc(int L, hipStream_t st) {
                    " is used for the first time inside a stream capture; run one call of "
                    "this shape (or create a plan) before capturing");
   }
-  d.tw = twiddle_table(dev, L);
-  d.stw = stage_table(dev, L);
-  return d;
+  FftDesc d{};
+  d.L = L;
+  int M = 0;
+  if (largest_prime_factor(L) > kBluesteinPrime) {
+    const int m = smooth_at_least(2 * L - 1);
+    if (m <= kBluesteinMaxM) M = m;
+  }
+  const int plan_len = M > 0 ? M : L;
+  d.nst = radix_plan_into(plan_len, d.rad, kMaxStages);
+  FC_REQUIRE(d.nst <= kMaxStages, kUnsupportedSize, "FFT plan too deep");
+  d.tw = twiddle_table(dev, plan_len);
+  if (M > 0) {
+    std::vector<float2> chirp, bfft;
+    bluestein_tables(L, M, &chirp, &bfft);
+    d.M = M;
+    d.chirp = upload_table(chirp);
+    d.bfft = upload_table(bfft);
+  } else {
+    d.stw = stage_table(dev, L);
+  }
+  std::lock_guard<std::mutex> lk(g_tw_mu);
+  auto ins = g_desc.emplace(key, d);
+  if (!ins.second) {  // another thread made it first: keep one
+    if (d.chirp) (void)hipFree(const_cast<float2*>(d.chirp));
+    if (d.bfft) (void)hipFree(const_cast<float2*>(d.bfft));
+  }
+  return ins.first->second;
 }
 
 // ---------------------------------------------------------------------------
@@ -352,7 +452,7 @@ ColArgs spectrum_column_args(const ConvShape& s, const FftDesc& fy, float2* S) {
   const long nxp = padded_bins(s.nx);
   ColArgs ya{};
   ya.fft = fy;
-  ya.tx = col_tile(s.ny, 2);
+  ya.tx = col_tile(fy.buf(), 2);
   ya.nbins = half_bins(s.nx);
   ya.src = S;
   ya.dst = S;
@@ -381,7 +481,7 @@ void pass_x_forward(const ConvShape& s, const float* src, float2* S, const float
       a.fft = fx;
       a.nrows = nrows;
       a.nbins = nb;
-      a.pairs_per_block = x_pairs_per_block(s.nx);
+      a.pairs_per_block = x_pairs_per_block(fx.buf());
       a.src = src;
       a.dst = S;
       a.src_row = s.nx;
@@ -420,7 +520,7 @@ void pass_y_forward(const ConvShape& s, float2* S, const float2* Kx, float2* Kxy
         PassScope pk(kPassKernelPlanes, st);
         ColArgs a{};
         a.fft = fy;
-        a.tx = col_tile(s.ny, 2);
+        a.tx = col_tile(fy.buf(), 2);
         a.nbins = nb;
         a.src = Kx;
         a.dst = Kxy;
@@ -449,7 +549,7 @@ void z_pass(const ConvShape& s, float2* S, const float2* operand, bool dense, bo
   PassScope ps(kPassZFused, st);
   ZArgs a{};
   a.fft = fft_desc(s.nz, st);
-  a.tx = col_tile(s.nz, 3);
+  a.tx = col_tile(a.fft.buf(), 3);
   a.nbins = half_bins(s.nx);
   a.s = S;
   a.s_row = nxp;
@@ -494,7 +594,7 @@ void pass_x_inverse(const ConvShape& s, const float2* S, float* dst, hipStream_t
     a.fft = fx;
     a.nrows = nrows;
     a.nbins = half_bins(s.nx);
-    a.pairs_per_block = x_pairs_per_block(s.nx);
+    a.pairs_per_block = x_pairs_per_block(fx.buf());
     a.csrc = S;
     a.rdst = dst;
     a.src_row = padded_bins(s.nx);

@@ -65,12 +65,21 @@ constexpr int kMaxStages = 24;
 
 // Device view of one 1D transform length: radix sequence (Stockham order) and
 // the FP64-generated, FP32-rounded table tw[j] = exp(-2*pi*i*j/L), j < L.
+// With M > 0 the length-L transform runs as Bluestein's chirp-z: a circular
+// convolution of length M >= 2L - 1 (7-smooth); rad/nst/tw are then M's
+// plan and table, chirp[n] = exp(-i pi n^2 / L) for n < L, and bfft the
+// length-M spectrum of the conjugate chirp sequence, scaled by 1/M.
 struct FftDesc {
   int L;
   int nst;
   int rad[kMaxStages];
   const float2* tw;   // runtime-length kernels
   const float2* stw;  // per-stage tables of the compile-time plan (or null)
+  int M;              // 0: plain Stockham plan of L
+  const float2* chirp;
+  const float2* bfft;
+  // positions a transform occupies in a tile buffer
+  __host__ __device__ int buf() const { return M > 0 ? M : L; }
 };
 
 }  // namespace fcamd

@@ -100,4 +100,43 @@ __device__ float2* gfft(float2* a, float2* b, const FftDesc& d, const TileGeom g
   return a;
 }
 
+// Whole transform of the tile whatever the plan: gfft, or for a Bluestein
+// descriptor (d.M > 0) the chirp-z form
+//   X[k] = c_k sum_n (x_n c_n) conj-chirp(k - n),  c = chirp (conj for DIR > 0)
+// as a circular convolution of length M: premultiply and zero-pad, FFT_M,
+// times bfft (conj for DIR > 0; the sequence is even), inverse FFT_M,
+// postmultiply.  The tile buffers hold d.buf() positions; the caller loads
+// positions [0, L) of `a`, and the result's positions [0, L) are valid.
+template <int DIR, bool CFAST>
+__device__ float2* gfft_any(float2* a, float2* b, const FftDesc& d, const TileGeom g) {
+  if (d.M == 0) return gfft<DIR, CFAST>(a, b, d, g);
+  const int L = d.L, M = d.M;
+  for (int w = threadIdx.x; w < M * g.C; w += blockDim.x) {
+    int c, p;
+    if (CFAST) { c = w % g.C; p = w / g.C; } else { p = w % M; c = w / M; }
+    float2& e = a[p * g.ps + c * g.cs];
+    e = p < L ? (DIR < 0 ? cmul(e, d.chirp[p]) : cmulc(e, d.chirp[p])) : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  FftDesc dm = d;
+  dm.L = M;
+  float2* r = gfft<-1, CFAST>(a, b, dm, g);
+  for (int w = threadIdx.x; w < M * g.C; w += blockDim.x) {
+    int c, k;
+    if (CFAST) { c = w % g.C; k = w / g.C; } else { k = w % M; c = w / M; }
+    float2& e = r[k * g.ps + c * g.cs];
+    e = DIR < 0 ? cmul(e, d.bfft[k]) : cmulc(e, d.bfft[k]);
+  }
+  __syncthreads();
+  r = gfft<+1, CFAST>(r, r == a ? b : a, dm, g);
+  for (int w = threadIdx.x; w < L * g.C; w += blockDim.x) {
+    int c, k;
+    if (CFAST) { c = w % g.C; k = w / g.C; } else { k = w % L; c = w / L; }
+    float2& e = r[k * g.ps + c * g.cs];
+    e = DIR < 0 ? cmul(e, d.chirp[k]) : cmulc(e, d.chirp[k]);
+  }
+  __syncthreads();
+  return r;
+}
+
 }  // namespace fcamd

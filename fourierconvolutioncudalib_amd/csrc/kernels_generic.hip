@@ -26,7 +26,7 @@ gk_xfwd(XPassArgs a) {
   extern __shared__ float2 smem[];
   const int L = a.fft.L;
   const int P = a.pairs_per_block;
-  const int LP = L + 1;  // odd row stride spreads rows over LDS banks
+  const int LP = a.fft.buf() + 1;  // odd row stride spreads rows over LDS banks
   float2* b0 = smem;
   float2* b1 = smem + (size_t)LP * P;
   const long pair0 = (long)blockIdx.x * P;
@@ -57,7 +57,7 @@ gk_xfwd(XPassArgs a) {
   }
   __syncthreads();
   const TileGeom g{P, 1, LP};
-  float2* z = gfft<-1, false>(b0, b1, a.fft, g);
+  float2* z = gfft_any<-1, false>(b0, b1, a.fft, g);
 
   const int nb = a.nbins;
   for (int w = threadIdx.x; w < nb * P; w += blockDim.x) {
@@ -84,7 +84,7 @@ gk_xinv(XPassArgs a) {
   extern __shared__ float2 smem[];
   const int L = a.fft.L;
   const int P = a.pairs_per_block;
-  const int LP = L + 1;
+  const int LP = a.fft.buf() + 1;
   float2* b0 = smem;
   float2* b1 = smem + (size_t)LP * P;
   const long pair0 = (long)blockIdx.x * P;
@@ -110,7 +110,7 @@ gk_xinv(XPassArgs a) {
   }
   __syncthreads();
   const TileGeom g{P, 1, LP};
-  float2* z = gfft<+1, false>(b0, b1, a.fft, g);
+  float2* z = gfft_any<+1, false>(b0, b1, a.fft, g);
 
   for (int w = threadIdx.x; w < L * P; w += blockDim.x) {
     const int pos = w % L, c = w / L;
@@ -141,7 +141,7 @@ __device__ void col_body(const ColArgs& a) {
   const int L = a.fft.L;
   const int TX = a.tx;
   float2* b0 = smem;
-  float2* b1 = smem + (size_t)L * TX;
+  float2* b1 = smem + (size_t)a.fft.buf() * TX;
   const int x0 = blockIdx.x * TX;
   const long o = blockIdx.y;
 
@@ -155,7 +155,7 @@ __device__ void col_body(const ColArgs& a) {
   }
   __syncthreads();
   const TileGeom g{TX, TX, 1};
-  float2* r = gfft<DIR, true>(b0, b1, a.fft, g);
+  float2* r = gfft_any<DIR, true>(b0, b1, a.fft, g);
   for (int w = threadIdx.x; w < L * TX; w += blockDim.x) {
     const int c = w % TX, pos = w / TX;
     if (x0 + c < a.nbins) a.dst[o * a.dst_outer + (long)pos * a.dst_elem + x0 + c] = r[pos * TX + c];
@@ -176,7 +176,7 @@ gk_zfused(ZArgs a) {
   extern __shared__ float2 smem[];
   const int L = a.fft.L;
   const int TX = a.tx;
-  const size_t tile = (size_t)L * TX;
+  const size_t tile = (size_t)a.fft.buf() * TX;
   float2* b0 = smem;
   float2* b1 = smem + tile;
   float2* b2 = smem + 2 * tile;
@@ -191,7 +191,7 @@ gk_zfused(ZArgs a) {
     b0[pos * TX + c] = v;
   }
   __syncthreads();
-  float2* A = gfft<-1, true>(b0, b1, a.fft, g);
+  float2* A = gfft_any<-1, true>(b0, b1, a.fft, g);
   float2* f0 = (A == b0) ? b1 : b0;  // the free image buffer
 
   for (int w = threadIdx.x; w < L * TX; w += blockDim.x) {
@@ -203,7 +203,7 @@ gk_zfused(ZArgs a) {
     f0[pos * TX + c] = v;
   }
   __syncthreads();
-  float2* K = gfft<-1, true>(f0, b2, a.fft, g);
+  float2* K = gfft_any<-1, true>(f0, b2, a.fft, g);
   float2* f1 = (K == f0) ? b2 : f0;
 
   for (int w = threadIdx.x; w < L * TX; w += blockDim.x) {
@@ -212,7 +212,7 @@ gk_zfused(ZArgs a) {
     A[w] = cscale(p, a.scale);
   }
   __syncthreads();
-  float2* R = gfft<+1, true>(A, f1, a.fft, g);
+  float2* R = gfft_any<+1, true>(A, f1, a.fft, g);
   for (int w = threadIdx.x; w < L * TX; w += blockDim.x) {
     const int c = w % TX, pos = w / TX;
     int q = pos;
@@ -229,19 +229,19 @@ gk_zfused(ZArgs a) {
 // launchers
 // ---------------------------------------------------------------------------
 void launch_gk_xfwd(const XPassArgs& a, long npairs, hipStream_t st) {
-  const size_t lds = 2 * sizeof(float2) * (size_t)(a.fft.L + 1) * a.pairs_per_block;
+  const size_t lds = 2 * sizeof(float2) * (size_t)(a.fft.buf() + 1) * a.pairs_per_block;
   const unsigned grid = (unsigned)((npairs + a.pairs_per_block - 1) / a.pairs_per_block);
   hipLaunchKernelGGL(gk_xfwd, dim3(grid), dim3(256), lds, st, a);
 }
 
 void launch_gk_xinv(const XPassArgs& a, long npairs, hipStream_t st) {
-  const size_t lds = 2 * sizeof(float2) * (size_t)(a.fft.L + 1) * a.pairs_per_block;
+  const size_t lds = 2 * sizeof(float2) * (size_t)(a.fft.buf() + 1) * a.pairs_per_block;
   const unsigned grid = (unsigned)((npairs + a.pairs_per_block - 1) / a.pairs_per_block);
   hipLaunchKernelGGL(gk_xinv, dim3(grid), dim3(256), lds, st, a);
 }
 
 void launch_gk_col(const ColArgs& a, int dir, long nouter, hipStream_t st) {
-  const size_t lds = 2 * sizeof(float2) * (size_t)a.fft.L * a.tx;
+  const size_t lds = 2 * sizeof(float2) * (size_t)a.fft.buf() * a.tx;
   const dim3 grid((a.nbins + a.tx - 1) / a.tx, (unsigned)nouter);
   if (dir < 0)
     hipLaunchKernelGGL(gk_col_fwd, grid, dim3(256), lds, st, a);
@@ -250,7 +250,7 @@ void launch_gk_col(const ColArgs& a, int dir, long nouter, hipStream_t st) {
 }
 
 void launch_gk_zfused(const ZArgs& a, long ny, hipStream_t st) {
-  const size_t lds = 3 * sizeof(float2) * (size_t)a.fft.L * a.tx;
+  const size_t lds = 3 * sizeof(float2) * (size_t)a.fft.buf() * a.tx;
   const dim3 grid((a.nbins + a.tx - 1) / a.tx, (unsigned)ny);
   hipLaunchKernelGGL(gk_zfused, grid, dim3(256), lds, st, a);
 }

@@ -30,7 +30,7 @@ __device__ void lcol_body(const LColArgs& a) {
   extern __shared__ float2 smem[];
   const int L = a.fft.L, TX = a.tx;
   float2* b0 = smem;
-  float2* b1 = smem + (size_t)L * TX;
+  float2* b1 = smem + (size_t)a.fft.buf() * TX;
   const long ntx = (a.ncols + TX - 1) / TX;
   const long q = blockIdx.x / ntx;
   const long x0 = (blockIdx.x % ntx) * TX;
@@ -51,7 +51,7 @@ __device__ void lcol_body(const LColArgs& a) {
   }
   __syncthreads();
   const TileGeom g{TX, TX, 1};
-  float2* r = gfft<DIR, true>(b0, b1, a.fft, g);
+  float2* r = gfft_any<DIR, true>(b0, b1, a.fft, g);
   for (int w = threadIdx.x; w < L * TX; w += blockDim.x) {
     const int c = w % TX, pos = w / TX;
     if (x0 + c >= a.ncols) continue;
@@ -181,7 +181,7 @@ void launch_lcol(const LColArgs& a, int dir, hipStream_t st) {
   const long ntx = (a.ncols + a.tx - 1) / a.tx;
   const long blocks = ntx * a.nouter;
   FC_REQUIRE(blocks >= 1 && blocks <= 0x7fffffffL, kUnsupportedSize, "too many column tiles");
-  const size_t lds = 2 * sizeof(float2) * (size_t)a.fft.L * a.tx;
+  const size_t lds = 2 * sizeof(float2) * (size_t)a.fft.buf() * a.tx;
   if (dir < 0)
     hipLaunchKernelGGL(lk_col_fwd, dim3((unsigned)blocks), dim3(256), lds, st, a);
   else

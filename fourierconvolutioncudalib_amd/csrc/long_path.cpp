@@ -51,7 +51,7 @@ LColArgs lcol(int L, long ncols, long nouter, float2* data, long hi, long lo, lo
               hipStream_t st) {
   LColArgs a{};
   a.fft = fft_desc(L, st);
-  a.tx = lcol_tile(L, ncols);
+  a.tx = lcol_tile(a.fft.buf(), ncols);
   a.ncols = ncols;
   a.nouter = nouter;
   a.src = data;

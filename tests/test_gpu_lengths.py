@@ -51,6 +51,24 @@ def test_runtime_length_kernels(fc, oracle, shape):
     _check(fc, oracle, shape, kshape, 7)
 
 
+# lengths whose largest prime factor exceeds 31 run Bluestein's chirp-z
+# (a circular convolution of 7-smooth length M >= 2L - 1, M <= 6826);
+# 4091 (M > 6826) keeps the direct prime stage
+@pytest.mark.parametrize("L", [37, 79, 109, 173, 346, 1009, 2018, 4091])
+@pytest.mark.parametrize("axis", ["x", "y", "z"])
+def test_bluestein_lengths(fc, oracle, axis, L):
+    from fourierconvolutioncudalib_amd import synth
+    shape = {"x": (3, 5, L), "y": (3, L, 6), "z": (L, 4, 6)}[axis]
+    kshape = tuple(min(n, k) for n, k in zip(shape, (3, 5, 7)))
+    print("bluestein", axis, L, flush=True)
+    img = synth.uniform(shape, 5000 + L)
+    k = synth.random_psf(kshape, 5001 + L)
+    for conj in (False, True):
+        out = fc.convolve(img, k, op=fc.CORRELATE if conj else fc.CONVOLVE)
+        ref = oracle.numpy_fft_conv(img, k, conj_kernel=conj)
+        assert oracle.rel_l2(out, ref) < TOL, (axis, L, conj)
+
+
 def test_static_matches_runtime_kernels_at_benchmark_shape(fc, oracle):
     import torch
     from fourierconvolutioncudalib_amd import synth
