@@ -293,13 +293,58 @@ __device__ __forceinline__ void sfft_for_outputs(int t, F f) {
   }
 }
 
+#ifndef FC_TW_COMPACT
+#define FC_TW_COMPACT 1
+#endif
+#ifndef FC_TW_COMPACT_MIN
+#define FC_TW_COMPACT_MIN 8
+#endif
 // Stage s twiddle rows of the thread's butterflies, loaded ahead of use.
+// Stages of radix 8 and above keep bases instead of the row's R - 1
+// entries: W^k, W^2k, W^3k (q[0].xy, q[0].zw, q[1].xy), W^4k (q[1].zw) and,
+// from radix 9, W^8k and W^12k (q[2]); entry r = r1 + 4*r2 is
+// W^(r1 k) * W^(4 r2 k).  That is one product more for the entries with
+// r1, r2 > 0, but fewer loads and registers (12 instead of 32 at radix 16):
+// one stage's twiddles are live beside the image's and the kernel column's
+// values at the fused z pass's register peak, and their loads sit between
+// the two barriers of every exchange.  Measured at 512 x 512 x 256: the z
+// pass 124 -> 115 us.
 template <int L, bool REV, int s>
 struct StageTw {
   using P = PlanOf<L, REV>;
   static constexpr int R = P::R(s), Ns = P::Ns(s), NB = P::NB(s), RP = even_up(R);
-  static constexpr int Q = Ns > 1 ? NB * (RP / 2) : 1;
+  static constexpr int CB = R >= 9 ? 3 : (R >= FC_TW_COMPACT_MIN ? 2 : 0);
+  static constexpr bool COMPACT = FC_TW_COMPACT && CB > 0;
+  // odd radices below that keep entries 1..R-1 only (the row's entry 0 is
+  // 1 and its last is padding): radix 3 in one float4 instead of two
+  static constexpr bool SHIFT = FC_TW_COMPACT && !COMPACT && (R & 1) && R >= 3;
+  static constexpr int QB = COMPACT ? CB : (SHIFT ? (R - 1) / 2 : RP / 2);  // float4 per butterfly
+  static constexpr int Q = Ns > 1 ? NB * QB : 1;
   float4 q[Q];
+  // twiddle of entry r (0 < r < R) of butterfly slot i
+  __device__ __forceinline__ float2 get(int i, int r) const {
+    const float4* b = &q[i * QB];
+    if constexpr (COMPACT) {
+      const int r1 = r & 3, r2 = r >> 2;
+      const float2 lo = r1 == 1 ? make_float2(b[0].x, b[0].y)
+                        : r1 == 2 ? make_float2(b[0].z, b[0].w)
+                                  : make_float2(b[1].x, b[1].y);
+      float2 hi = make_float2(b[1].z, b[1].w);  // r2 == 1
+      if constexpr (CB == 3) {
+        if (r2 == 2) hi = make_float2(b[2].x, b[2].y);
+        if (r2 == 3) hi = make_float2(b[2].z, b[2].w);
+      }
+      if (r2 == 0) return lo;
+      if (r1 == 0) return hi;
+      return cmul(lo, hi);
+    } else if constexpr (SHIFT) {
+      const float4 p = b[(r - 1) / 2];
+      return ((r - 1) & 1) ? make_float2(p.z, p.w) : make_float2(p.x, p.y);
+    } else {
+      const float4 p = b[r / 2];
+      return (r & 1) ? make_float2(p.z, p.w) : make_float2(p.x, p.y);
+    }
+  }
 };
 
 template <int L, bool REV, int s>
@@ -313,43 +358,36 @@ __device__ __forceinline__ void sfft_load_tw(StageTw<L, REV, s>& tw, int t,
     for (int i = 0; i < W::NB; ++i) {
       const int b = t + i * P::T;
       if ((M % P::T) != 0 && b >= M) continue;
-      const float4* q = reinterpret_cast<const float4*>(stw + P::TWOFF(s) + (b % W::Ns) * W::RP);
+      const float2* row = stw + P::TWOFF(s) + (b % W::Ns) * W::RP;
+      if constexpr (W::COMPACT) {
+        // rows are an even number of float2 apart, so entries 2-3 form an
+        // aligned float4
+        const float2 e1 = row[1], e4 = row[4];
+        const float4 e23 = *reinterpret_cast<const float4*>(row + 2);
+        tw.q[i * W::QB + 0] = make_float4(e1.x, e1.y, e23.x, e23.y);
+        tw.q[i * W::QB + 1] = make_float4(e23.z, e23.w, e4.x, e4.y);
+        if constexpr (W::CB == 3) {
+          const float2 e8 = row[8];
+          const float2 e12 = W::R > 12 ? row[12] : make_float2(1.f, 0.f);
+          tw.q[i * W::QB + 2] = make_float4(e8.x, e8.y, e12.x, e12.y);
+        }
+      } else if constexpr (W::SHIFT) {
 #pragma unroll
-      for (int r2 = 0; r2 < W::RP / 2; ++r2) tw.q[i * (W::RP / 2) + r2] = q[r2];
-    }
-  }
-}
-
-// sfft_compute with the twiddles already in registers
-template <int L, bool REV, int DIR, int s, int MAXE>
-__device__ __forceinline__ void sfft_compute_tw(float2 (&v)[MAXE], int t,
-                                                const StageTw<L, REV, s>& tw) {
-  using P = PlanOf<L, REV>;
-  using W = StageTw<L, REV, s>;
-  constexpr int R = P::R(s), M = P::M(s), NB = P::NB(s);
+        for (int m = 0; m < W::QB; ++m) {
+          const float2 ea = row[2 * m + 1], eb = row[2 * m + 2];
+          tw.q[i * W::QB + m] = make_float4(ea.x, ea.y, eb.x, eb.y);
+        }
+      } else {
+        const float4* q = reinterpret_cast<const float4*>(row);
 #pragma unroll
-  for (int i = 0; i < NB; ++i) {
-    const int b = t + i * P::T;
-    if ((M % P::T) != 0 && b >= M) continue;
-    float2* x = &v[i * R];
-    if constexpr (W::Ns > 1) {
-      float2 w[W::RP];
-#pragma unroll
-      for (int r2 = 0; r2 < W::RP / 2; ++r2) {
-        const float4 p = tw.q[i * (W::RP / 2) + r2];
-        w[2 * r2] = make_float2(p.x, p.y);
-        w[2 * r2 + 1] = make_float2(p.z, p.w);
+        for (int r2 = 0; r2 < W::RP / 2; ++r2) tw.q[i * (W::RP / 2) + r2] = q[r2];
       }
-#pragma unroll
-      for (int r = 1; r < R; ++r) x[r] = DIR < 0 ? cmul(x[r], w[r]) : cmulc(x[r], w[r]);
     }
-    Dft<R, DIR>::run(x);
   }
 }
 
-// The two halves of sfft_compute_tw: the twiddle multiply, then the
-// butterflies (lets two transforms share one twiddle set that dies before
-// either butterfly runs).
+// The twiddle multiply of one stage (lets two transforms share one twiddle
+// set that dies before either butterfly runs).
 template <int L, bool REV, int DIR, int s, int MAXE>
 __device__ __forceinline__ void sfft_twiddle(float2 (&v)[MAXE], int t, const StageTw<L, REV, s>& tw) {
   using P = PlanOf<L, REV>;
@@ -362,13 +400,26 @@ __device__ __forceinline__ void sfft_twiddle(float2 (&v)[MAXE], int t, const Sta
       if ((M % P::T) != 0 && b >= M) continue;
       float2* x = &v[i * R];
 #pragma unroll
-      for (int r2 = 0; r2 < W::RP / 2; ++r2) {
-        const float4 p = tw.q[i * (W::RP / 2) + r2];
-        const int r0 = 2 * r2, r1 = 2 * r2 + 1;
-        if (r0 > 0 && r0 < R) x[r0] = DIR < 0 ? cmul(x[r0], make_float2(p.x, p.y)) : cmulc(x[r0], make_float2(p.x, p.y));
-        if (r1 < R) x[r1] = DIR < 0 ? cmul(x[r1], make_float2(p.z, p.w)) : cmulc(x[r1], make_float2(p.z, p.w));
+      for (int r = 1; r < R; ++r) {
+        const float2 w = tw.get(i, r);
+        x[r] = DIR < 0 ? cmul(x[r], w) : cmulc(x[r], w);
       }
     }
+  }
+}
+
+// sfft_compute with the twiddles already in registers
+template <int L, bool REV, int DIR, int s, int MAXE>
+__device__ __forceinline__ void sfft_compute_tw(float2 (&v)[MAXE], int t,
+                                                const StageTw<L, REV, s>& tw) {
+  using P = PlanOf<L, REV>;
+  constexpr int R = P::R(s), M = P::M(s), NB = P::NB(s);
+  sfft_twiddle<L, REV, DIR, s>(v, t, tw);
+#pragma unroll
+  for (int i = 0; i < NB; ++i) {
+    const int b = t + i * P::T;
+    if ((M % P::T) != 0 && b >= M) continue;
+    Dft<R, DIR>::run(&v[i * R]);
   }
 }
 

@@ -20,6 +20,24 @@
 
 namespace fcamd {
 
+// minimum waves per SIMD asked of the two register-heaviest kernels (0: no
+// request; the compiler then sizes registers for the code alone).  4 waves
+// (the most their LDS allows) took the 10-iteration, 4-view 384 x 384 x 192
+// run from 15.34 to 15.04 ms, although the ratio kernel then spills 8
+// registers.
+#ifndef FC_RL_XRATIO_WAVES
+#define FC_RL_XRATIO_WAVES 4
+#endif
+#ifndef FC_RL_ZSUM_WAVES
+#define FC_RL_ZSUM_WAVES 4
+#endif
+#define FC_RL_BOUNDS_0(n) __launch_bounds__(n)
+#define FC_RL_BOUNDS_3(n) __launch_bounds__(n, 3)
+#define FC_RL_BOUNDS_4(n) __launch_bounds__(n, 4)
+#define FC_RL_BOUNDS_5(n) __launch_bounds__(n, 5)
+#define FC_RL_BOUNDS_CAT(n, w) FC_RL_BOUNDS_##w(n)
+#define FC_RL_BOUNDS(n, w) FC_RL_BOUNDS_CAT(n, w)
+
 // loads the compact kernel column of view v (planes placed at (z - cz) mod
 // L); positions are t + (compile-time constant) after unrolling, so each
 // access is a per-thread base plus a uniform offset
@@ -35,6 +53,14 @@ __device__ __forceinline__ void load_kernel_column(float2 (&w)[MAXE], int t, boo
     if (ok && j < kz) val = (wrap ? kw : kt)[(long)(pos - t) * kplane];
     w[slot] = val;
   });
+}
+
+// phi / d for d = max(blur, eps) > 0: the hardware reciprocal (at most
+// 1 ulp) and one product, one rounding more than a correctly rounded
+// division and some ten instructions fewer per sample; a subnormal d
+// (only for a subnormal eps) keeps the division
+__device__ __forceinline__ float rl_div(float phi, float d) {
+  return d >= 1.17549435e-38f ? phi * __builtin_amdgcn_rcpf(d) : phi / d;
 }
 
 // column access at position pos through a base at position t
@@ -75,7 +101,7 @@ sk_zfan(RLZArgs a) {
 }
 
 template <int L, int TX>
-__global__ void __launch_bounds__(TX * threads_per_transform<L>())
+__global__ void FC_RL_BOUNDS(TX * threads_per_transform<L>(), FC_RL_ZSUM_WAVES)
 sk_zsum(RLZArgs a) {
   using P = PlanOf<L, false>;
   __shared__ float2 lds[L * TX];
@@ -151,8 +177,8 @@ sk_xratio(RLXArgs a) {
   const float eps = a.eps;
   sfft_for_outputs<L, true>(t, [&](int slot, int pos) {
     float ya = 0.f, yb = 0.f;
-    if (oka) ya = fa[pos] / fmaxf(z[slot].x, eps);
-    if (okb) yb = fb[pos] / fmaxf(z[slot].y, eps);
+    if (oka) ya = rl_div(fa[pos], fmaxf(z[slot].x, eps));
+    if (okb) yb = rl_div(fb[pos], fmaxf(z[slot].y, eps));
     z[slot] = make_float2(ya, yb);
   });
   sfft_run<L, false, -1>(z, t, row, addr, a.fft.stw);
@@ -212,7 +238,7 @@ sk_xupdate(RLXArgs a) {
 // the flat x passes (xflat_* in static_kernels_impl.hpp).  Real rows must be
 // dense (rrow == L).
 template <int L, int PAIRS>
-__global__ void __launch_bounds__(PAIRS * threads_per_transform<L>())
+__global__ void FC_RL_BOUNDS(PAIRS * threads_per_transform<L>(), FC_RL_XRATIO_WAVES)
 sk_xratio_flat(RLXArgs a) {
   using P = PlanOf<L, false>;
   constexpr int T = P::T, LP = XRow<L>::LP, SL = XStage<L>::SL, NB = XStage<L>::NB;
@@ -256,8 +282,8 @@ sk_xratio_flat(RLXArgs a) {
   __syncthreads();
   const float eps = a.eps;
   sfft_for_outputs<L, true>(t, [&](int slot, int pos) {
-    const float ya = oka ? st[2 * c * SL + pos] / fmaxf(z[slot].x, eps) : 0.f;
-    const float yb = okb ? st[(2 * c + 1) * SL + pos] / fmaxf(z[slot].y, eps) : 0.f;
+    const float ya = oka ? rl_div(st[2 * c * SL + pos], fmaxf(z[slot].x, eps)) : 0.f;
+    const float yb = okb ? rl_div(st[(2 * c + 1) * SL + pos], fmaxf(z[slot].y, eps)) : 0.f;
     z[slot] = make_float2(ya, yb);
   });
   __syncthreads();
