@@ -604,6 +604,43 @@ void pass_x_inverse(const ConvShape& s, const float2* S, float* dst, hipStream_t
   }
 }
 
+namespace {
+PlaneArgs plane_args(const ConvShape& s, hipStream_t st) {
+  PlaneArgs a{};
+  a.fx = fft_desc(s.nx, st);
+  a.fy = fft_desc(s.ny, st);
+  a.nz = s.nz;
+  a.nxp = padded_bins(s.nx);
+  return a;
+}
+}  // namespace
+
+void plane_forward(const ConvShape& s, const float* src, float2* S, const float* kernel,
+                   float2* Kxy, hipStream_t st) {
+  ensure_kernel_attributes();
+  PlaneArgs a = plane_args(s, st);
+  a.src = src;
+  a.spec = S;
+  a.kernel = kernel;
+  a.kplanes = Kxy;
+  a.kx = s.kx;
+  a.ky = s.ky;
+  a.kz = s.kz;
+  a.cx = s.cx;
+  a.cy = s.cy;
+  PassScope ps(kPassXForward, st);
+  plane_launch(s, a, -1, st);
+}
+
+void plane_inverse(const ConvShape& s, const float2* S, float* dst, hipStream_t st) {
+  ensure_kernel_attributes();
+  PlaneArgs a = plane_args(s, st);
+  a.spec = const_cast<float2*>(S);
+  a.dst = dst;
+  PassScope ps(kPassXInverse, st);
+  plane_launch(s, a, +1, st);
+}
+
 int fast_length(int n) {
   FC_REQUIRE(n >= 1, kInvalidArgument, "transform length must be positive");
   int smooth = n;  // smallest 7-smooth length >= n
@@ -679,9 +716,15 @@ void enqueue_convolution(const ConvShape& s, float* d_out, const float* d_image,
   float2* S = reinterpret_cast<float2*>(base);
   float2* Kx = reinterpret_cast<float2*>(base + wl.spectrum);
   float2* Kxy = reinterpret_cast<float2*>(base + wl.spectrum + wl.krows);
-  forward_xy(s, d_image, S, d_kernel, Kx, Kxy, st);
-  z_pass(s, S, Kxy, false, op == Op::kCorrelate, st);
-  inverse_yx(s, S, d_out, st);
+  if (plane_path_available(s)) {
+    plane_forward(s, d_image, S, d_kernel, Kxy, st);
+    z_pass(s, S, Kxy, false, op == Op::kCorrelate, st);
+    plane_inverse(s, S, d_out, st);
+  } else {
+    forward_xy(s, d_image, S, d_kernel, Kx, Kxy, st);
+    z_pass(s, S, Kxy, false, op == Op::kCorrelate, st);
+    inverse_yx(s, S, d_out, st);
+  }
   FC_HIP_CHECK(hipGetLastError());
 }
 

@@ -119,6 +119,15 @@ void z_pass(const ConvShape& s, float2* S, const float2* operand, bool dense, bo
 void inverse_yx(const ConvShape& s, float2* S, float* dst, hipStream_t stream,
                 const VPad* vp = nullptr);
 
+// Whole-plane x and y transforms (plane_kernels.hip) for plane extents that
+// fit one workgroup's LDS: forward_xy's image spectrum and kernel planes (no
+// kernel rows) in one launch, inverse_yx in one launch.
+bool plane_path_available(const ConvShape& s);
+void plane_launch(const ConvShape& s, const PlaneArgs& a, int dir, hipStream_t stream);
+void plane_forward(const ConvShape& s, const float* src, float2* S, const float* kernel,
+                   float2* Kxy, hipStream_t stream);
+void plane_inverse(const ConvShape& s, const float2* S, float* dst, hipStream_t stream);
+
 // Boundary-mode convolution (FCAMD_BOUNDARY_*): the circular convolution
 // of a virtually padded volume, cropped.  Each axis d with k_d > 1 is padded
 // to a fast length P_d >= n_d + k_d - 1 with the image placed at

@@ -140,6 +140,20 @@ struct ZArgs {
   int out_shift;
 };
 
+// Whole-plane x and y transforms (plane_kernels.hpp): one workgroup per
+// z-plane of the image, then one per plane of the compact kernel.
+struct PlaneArgs {
+  FftDesc fx, fy;
+  int nz;               // image planes (the first nz workgroups)
+  long nxp;             // spectrum row pitch (bins)
+  const float* src;     // forward: image planes [nz][ny][nx]
+  float2* spec;         // spectrum planes [nz][ny][nxp]
+  const float* kernel;  // forward: compact kernel [kz][ky][kx], or null
+  float2* kplanes;      // kernel planes [kz][ny][nxp]
+  int kx, ky, kz, cx, cy;
+  float* dst;  // inverse: real planes [nz][ny][nx]
+};
+
 // Column tiles of TX consecutive bins over `nouter` rows.  When the bins end
 // in a single trailing bin (even nx: bin nx/2 after full tiles), that bin is
 // not given a nearly empty tile per row; instead "Nyquist tiles" gather it

@@ -1,0 +1,82 @@
+"""Whole-plane x/y kernels (plane_kernels.hpp): for plane extents with
+compile-time plane kernels the forward x-R2C and y-FFT run as one launch per
+z-plane (plus one per kernel plane) and the inverse y-IFFT and x-C2R as
+one, around the unchanged fused z pass.  Checked against the FP64 oracle at
+rel-L2 <= 1e-5 (the bar of every convolution here) for every instantiated
+plane extent, convolution and correlation, kernels from one tap to the full
+plane, off-centre and even extents, through the drop-in host entry point
+and the device entry point; the per-pass event timing confirms that no
+separate y pass was launched.  Semantics: src/convolution3Dfft.cu:399-578
+(circular convolution, kernel centred at floor(k/2)).
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+# (image C-order shape, kernel C-order shape)
+CASES = [((64, 128, 128), (15, 15, 15)),   # the 128 x 128 x 64 workload
+         ((5, 128, 128), (3, 128, 128)),    # kernel as wide as the plane
+         ((33, 64, 64), (1, 1, 1)),         # one tap
+         ((17, 96, 96), (4, 6, 9)),         # even kernel extents (centre floor(k/2))
+         ((12, 64, 128), (7, 13, 5)),
+         ((20, 128, 64), (9, 3, 11)),
+         ((1, 128, 128), (1, 7, 5))]        # one plane
+
+
+def _ids(v):
+    return "x".join(map(str, v))
+
+
+@pytest.mark.parametrize("shape,kshape", CASES, ids=_ids)
+def test_plane_extents(fc, oracle, shape, kshape):
+    from fourierconvolutioncudalib_amd import synth
+    print("plane case", shape, kshape, flush=True)
+    stack = synth.uniform(shape, 7100 + sum(shape))
+    kernel = synth.random_psf(kshape, 7200 + sum(kshape))
+    im = np.ascontiguousarray(stack, dtype=np.float32).copy()
+    k = np.ascontiguousarray(kernel, dtype=np.float32)
+    fc.convolution3DfftCUDAInPlace(im.reshape(-1), list(im.shape[::-1]), k.reshape(-1),
+                                   list(k.shape[::-1]), 0)
+    ref = oracle.numpy_fft_conv(stack, kernel)
+    assert oracle.rel_l2(im, ref) < TOL
+    import torch
+    ti = torch.from_numpy(stack).cuda()
+    tk = torch.from_numpy(kernel).cuda()
+    got = fc.convolve3d(ti, tk, op=fc.CORRELATE).cpu().numpy()
+    ref_c = oracle.numpy_fft_conv(stack, kernel, conj_kernel=True)
+    assert oracle.rel_l2(got, ref_c) < TOL
+
+
+def test_plane_path_runs(fc, oracle):
+    """One launch forward and one inverse around the z pass: the per-pass
+    timing records no y pass for a plane extent, and records it for a shape
+    without plane kernels."""
+    import torch
+    from fourierconvolutioncudalib_amd import synth
+    lib = fc.lib()
+
+    def passes(shape, kshape):
+        img = torch.from_numpy(synth.uniform(shape, 1)).cuda()
+        k = torch.from_numpy(synth.random_psf(kshape, 2)).cuda()
+        fc.convolve3d(img, k)
+        torch.cuda.synchronize()
+        lib.fcamd_pass_timing(1)
+        fc.convolve3d(img, k)
+        torch.cuda.synchronize()
+        lib.fcamd_pass_timing(0)
+        ms = (ctypes.c_double * 16)()
+        cnt = (ctypes.c_int * 16)()
+        n = lib.fcamd_pass_times(ms, cnt, 16)
+        return {lib.fcamd_pass_name(i).decode(): cnt[i] for i in range(n)}
+
+    plane = passes((16, 128, 128), (5, 5, 5))
+    print("plane path launches", plane, flush=True)
+    assert plane["image_xfwd"] == 1 and plane["image_xinv"] == 1 and plane["zfused"] == 1
+    assert plane["image_yfwd"] == 0 and plane["image_yinv"] == 0
+    other = passes((16, 80, 80), (5, 5, 5))
+    assert other["image_yfwd"] == 1 and other["image_yinv"] == 1
