@@ -309,11 +309,11 @@ __device__ __forceinline__ void sfft_for_outputs(int t, F f) {
 // values at the fused z pass's register peak, and their loads sit between
 // the two barriers of every exchange.  Measured at 512 x 512 x 256: the z
 // pass 124 -> 115 us.
-template <int L, bool REV, int s>
+template <int L, bool REV, int s, int CM = FC_TW_COMPACT_MIN>
 struct StageTw {
   using P = PlanOf<L, REV>;
   static constexpr int R = P::R(s), Ns = P::Ns(s), NB = P::NB(s), RP = even_up(R);
-  static constexpr int CB = R >= 9 ? 3 : (R >= FC_TW_COMPACT_MIN ? 2 : 0);
+  static constexpr int CB = R >= 9 && R >= CM ? 3 : (R >= CM ? 2 : 0);
   static constexpr bool COMPACT = FC_TW_COMPACT && CB > 0;
   // odd radices below that keep entries 1..R-1 only (the row's entry 0 is
   // 1 and its last is padding): radix 3 in one float4 instead of two
@@ -347,11 +347,11 @@ struct StageTw {
   }
 };
 
-template <int L, bool REV, int s>
-__device__ __forceinline__ void sfft_load_tw(StageTw<L, REV, s>& tw, int t,
+template <int L, bool REV, int s, int CM>
+__device__ __forceinline__ void sfft_load_tw(StageTw<L, REV, s, CM>& tw, int t,
                                              const float2* __restrict__ stw) {
   using P = PlanOf<L, REV>;
-  using W = StageTw<L, REV, s>;
+  using W = StageTw<L, REV, s, CM>;
   if constexpr (W::Ns > 1) {
     constexpr int M = P::M(s);
 #pragma unroll
@@ -388,10 +388,11 @@ __device__ __forceinline__ void sfft_load_tw(StageTw<L, REV, s>& tw, int t,
 
 // The twiddle multiply of one stage (lets two transforms share one twiddle
 // set that dies before either butterfly runs).
-template <int L, bool REV, int DIR, int s, int MAXE>
-__device__ __forceinline__ void sfft_twiddle(float2 (&v)[MAXE], int t, const StageTw<L, REV, s>& tw) {
+template <int L, bool REV, int DIR, int s, int MAXE, int CM>
+__device__ __forceinline__ void sfft_twiddle(float2 (&v)[MAXE], int t,
+                                             const StageTw<L, REV, s, CM>& tw) {
   using P = PlanOf<L, REV>;
-  using W = StageTw<L, REV, s>;
+  using W = StageTw<L, REV, s, CM>;
   constexpr int R = P::R(s), M = P::M(s), NB = P::NB(s);
   if constexpr (W::Ns > 1) {
 #pragma unroll
@@ -409,9 +410,9 @@ __device__ __forceinline__ void sfft_twiddle(float2 (&v)[MAXE], int t, const Sta
 }
 
 // sfft_compute with the twiddles already in registers
-template <int L, bool REV, int DIR, int s, int MAXE>
+template <int L, bool REV, int DIR, int s, int MAXE, int CM>
 __device__ __forceinline__ void sfft_compute_tw(float2 (&v)[MAXE], int t,
-                                                const StageTw<L, REV, s>& tw) {
+                                                const StageTw<L, REV, s, CM>& tw) {
   using P = PlanOf<L, REV>;
   constexpr int R = P::R(s), M = P::M(s), NB = P::NB(s);
   sfft_twiddle<L, REV, DIR, s>(v, t, tw);
@@ -440,28 +441,31 @@ __device__ __forceinline__ void sfft_butterflies(float2 (&v)[MAXE], int t) {
 // twiddles are requested before the barriers of its exchange (the barrier
 // waits for LDS traffic only), so their latency overlaps the exchange; the
 // data registers are free at that point.
-template <int L, bool REV, int DIR, int s, int MAXE, class Addr>
+template <int L, bool REV, int DIR, int s, int CM, int MAXE, class Addr>
 __device__ __forceinline__ void sfft_rest(float2 (&v)[MAXE], int t, float2* lds, Addr addr,
                                           const float2* __restrict__ tw) {
   using P = PlanOf<L, REV>;
   if constexpr (s < P::S) {
     sfft_store_lds<L, REV, s - 1>(v, t, lds, addr);
-    StageTw<L, REV, s> stw;
+    StageTw<L, REV, s, CM> stw;
     sfft_load_tw<L, REV, s>(stw, t, tw);
     __syncthreads();
     sfft_load_lds<L, REV, s>(v, t, lds, addr);
     __syncthreads();
     sfft_compute_tw<L, REV, DIR, s>(v, t, stw);
-    sfft_rest<L, REV, DIR, s + 1>(v, t, lds, addr, tw);
+    sfft_rest<L, REV, DIR, s + 1, CM>(v, t, lds, addr, tw);
   }
 }
 
-// Whole transform: stage 0 compute, then the remaining stages.
-template <int L, bool REV, int DIR, int MAXE, class Addr>
+// Whole transform: stage 0 compute, then the remaining stages.  CM is the
+// smallest radix whose stages keep compact twiddles (StageTw); the x-R2C
+// passes measured faster with full rows at radix 8 (512 x 512 x 256: 105
+// against 110 us) and with compact ones at radix 16 (256 x 256 x 128).
+template <int L, bool REV, int DIR, int CM = FC_TW_COMPACT_MIN, int MAXE, class Addr>
 __device__ __forceinline__ void sfft_run(float2 (&v)[MAXE], int t, float2* lds, Addr addr,
                                          const float2* __restrict__ tw) {
   sfft_compute<L, REV, DIR, 0>(v, t, tw);
-  sfft_rest<L, REV, DIR, 1>(v, t, lds, addr, tw);
+  sfft_rest<L, REV, DIR, 1, CM>(v, t, lds, addr, tw);
 }
 
 }  // namespace fcamd

@@ -92,7 +92,7 @@ __device__ __forceinline__ void xfwd_body(const XPassArgs& a, long blk, float2* 
   });
   float2* row = lds + c * LP;
   auto addr = [&](int pos) { return XRow<L>::at(pos); };
-  sfft_run<L, false, -1>(v, t, row, addr, a.fft.stw);
+  sfft_run<L, false, -1, 9>(v, t, row, addr, a.fft.stw);
   sfft_for_outputs<L, false>(t, [&](int slot, int pos) { row[addr(pos)] = v[slot]; });
   __syncthreads();
   float2* da = a.dst + ra * a.dst_row;
@@ -132,7 +132,7 @@ __device__ __forceinline__ void xfwd_vpad_body(const XPassArgs& a, long blk, flo
   });
   float2* row = lds + c * LP;
   auto addr = [&](int pos) { return XRow<L>::at(pos); };
-  sfft_run<L, false, -1>(v, t, row, addr, a.fft.stw);
+  sfft_run<L, false, -1, 9>(v, t, row, addr, a.fft.stw);
   sfft_for_outputs<L, false>(t, [&](int slot, int pos) { row[addr(pos)] = v[slot]; });
   __syncthreads();
   float2* da = a.dst + ra * a.dst_row;
@@ -435,7 +435,7 @@ __device__ __forceinline__ void xfwd_flat_body(const XPassArgs& a, long blk, flo
   __syncthreads();
   float2* row = lds + c * LP;
   auto addr = [&](int pos) { return XRow<L>::at(pos); };
-  sfft_run<L, false, -1>(v, t, row, addr, a.fft.stw);
+  sfft_run<L, false, -1, 9>(v, t, row, addr, a.fft.stw);
   sfft_for_outputs<L, false>(t, [&](int slot, int pos) { row[addr(pos)] = v[slot]; });
   __syncthreads();
   // Hermitian split, bin-contiguous over the workgroup's rows

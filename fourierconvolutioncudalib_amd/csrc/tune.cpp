@@ -4,12 +4,12 @@
 // more than one (x-R2C row pairs, x-C2R row pairs, y tile shape / pipelining
 // / radix order, z kernel-spectrum method) with the others fixed, keeps the
 // fastest, and moves to the next pass (coordinate descent over passes).  A
-// variant is timed by the per-pass events of the launches it runs.  The
+// variant is timed by whole convolutions, the quantity callers see.  The
 // choice is per (pass, transform length) and process-wide.  With
 // FCAMD_TUNE_CACHE set, choices are appended to that file as
 // "pass length variant-name milliseconds" lines (milliseconds per
-// convolution in that pass's launches) and re-applied by
-// fcamd_autotune_load() (called on the first tuning request).
+// convolution) and re-applied by fcamd_autotune_load() (called on the first
+// tuning request).
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
@@ -18,6 +18,7 @@
 #include <mutex>
 #include <sstream>
 #include <string>
+#include <vector>
 
 #include "../../include/fourierconvolution_amd.h"
 #include "conv_driver.hpp"
@@ -43,35 +44,26 @@ void apply_cached(const std::string& path) {
   }
 }
 
-// Milliseconds per convolution spent in the launches that run `pass`'s
-// variant (x forward, x inverse, both y passes, or the fused z pass), from
-// the per-pass events over `reps` convolutions; the best of three rounds.
-// Timing only the affected passes keeps the other passes' run-to-run noise
-// (a few microseconds each) out of the comparison.
-float time_variant(const ConvShape& s, int pass, float* out, const float* img, const float* ker,
-                   void* ws, hipStream_t st, int reps) {
-  for (int i = 0; i < 2; ++i) enqueue_convolution(s, out, img, ker, Op::kConvolve, ws, st);
-  FC_HIP_CHECK(hipStreamSynchronize(st));
-  double ms[kNumPasses];
-  int cnt[kNumPasses];
-  pass_timing_collect(ms, cnt);  // drops events recorded before
-  float best = 1e30f;
-  for (int round = 0; round < 3; ++round) {
-    pass_timing_enable(true);
-    for (int i = 0; i < reps; ++i) enqueue_convolution(s, out, img, ker, Op::kConvolve, ws, st);
-    pass_timing_enable(false);
-    FC_HIP_CHECK(hipStreamSynchronize(st));
-    pass_timing_collect(ms, cnt);
-    double t = 0.0;
-    switch (pass) {
-      case kSxFwd: t = ms[kPassXForward]; break;
-      case kSxInv: t = ms[kPassXInverse]; break;
-      case kSCol: t = ms[kPassYForward] + ms[kPassYInverse]; break;
-      default: t = ms[kPassZFused]; break;
-    }
-    best = std::min(best, (float)(t / reps));
-  }
-  return best;
+// Milliseconds per whole convolution over `reps` back-to-back calls, one
+// event pair around the block (per-pass events would space the launches
+// apart and change what each pass finds in the caches the previous one
+// filled).
+float time_block(const ConvShape& s, float* out, const float* img, const float* ker, void* ws,
+                 hipStream_t st, int reps, hipEvent_t e0, hipEvent_t e1) {
+  enqueue_convolution(s, out, img, ker, Op::kConvolve, ws, st);  // warm-up
+  FC_HIP_CHECK(hipEventRecord(e0, st));
+  for (int i = 0; i < reps; ++i) enqueue_convolution(s, out, img, ker, Op::kConvolve, ws, st);
+  FC_HIP_CHECK(hipEventRecord(e1, st));
+  FC_HIP_CHECK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  FC_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+  return ms / reps;
+}
+
+float median(std::vector<float> v) {
+  std::sort(v.begin(), v.end());
+  const size_t n = v.size();
+  return n % 2 ? v[n / 2] : 0.5f * (v[n / 2 - 1] + v[n / 2]);
 }
 
 template <class F>
@@ -122,11 +114,14 @@ int fcamd_autotune(const int* imDim, const int* kernelDim, int reps, void* strea
     const size_t nws = workspace_layout(s).total();
     float *img = nullptr, *out = nullptr, *ker = nullptr;
     void* ws = nullptr;
-    // per-pass events are the tuner's clock; a caller's own pass timing is
-    // suspended meanwhile (its pending events are consumed) and restored
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    // a caller's per-pass timing is suspended meanwhile and restored
     const bool timing = pass_timing_enabled();
+    pass_timing_enable(false);
     auto cleanup = [&] {
       pass_timing_enable(timing);
+      if (e0) (void)hipEventDestroy(e0);
+      if (e1) (void)hipEventDestroy(e1);
       if (img) (void)hipFree(img);
       if (out) (void)hipFree(out);
       if (ker) (void)hipFree(ker);
@@ -137,21 +132,34 @@ int fcamd_autotune(const int* imDim, const int* kernelDim, int reps, void* strea
       FC_HIP_CHECK(hipMalloc(&out, nimg));
       FC_HIP_CHECK(hipMalloc(&ker, nker));
       FC_HIP_CHECK(hipMalloc(&ws, nws));
+      FC_HIP_CHECK(hipEventCreate(&e0));
+      FC_HIP_CHECK(hipEventCreate(&e1));
       // contents do not affect the timing; a fixed byte pattern keeps the
       // values finite (0x3c3c3c3c ~ 0.0115f)
       FC_HIP_CHECK(hipMemsetAsync(img, 0x3c, nimg, st));
       FC_HIP_CHECK(hipMemsetAsync(ker, 0x3c, nker, st));
       const int axis_len[kSNumPasses] = {s.nx, s.nx, s.ny, s.nz};
       const char* cache = std::getenv("FCAMD_TUNE_CACHE");
+      // variants of one pass are timed in interleaved rounds (drift and
+      // clock changes hit all of them alike) and ranked by their median;
+      // the current choice stays unless another is faster by more than
+      // 0.5 %, the spread of repeated measurements
+      constexpr int kRounds = 5;
       for (int pass = 0; pass < kSNumPasses; ++pass) {
         StaticEntry* e = static_lookup(axis_len[pass]);
         if (!e || e->count(pass) < 2) continue;
-        int best = e->selected[pass];
-        float best_ms = 1e30f;
-        for (int v = 0; v < e->count(pass); ++v) {
-          e->selected[pass] = v;
-          const float ms = time_variant(s, pass, out, img, ker, ws, st, reps);
-          if (ms < best_ms) {
+        const int n = e->count(pass), current = e->selected[pass];
+        std::vector<std::vector<float>> t(n);
+        for (int round = 0; round < kRounds; ++round)
+          for (int v = 0; v < n; ++v) {
+            e->selected[pass] = v;
+            t[v].push_back(time_block(s, out, img, ker, ws, st, reps, e0, e1));
+          }
+        int best = current;
+        float best_ms = median(t[current]);
+        for (int v = 0; v < n; ++v) {
+          const float ms = median(t[v]);
+          if (ms < best_ms * 0.995f) {
             best_ms = ms;
             best = v;
           }
