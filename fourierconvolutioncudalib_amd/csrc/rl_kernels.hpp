@@ -16,6 +16,8 @@
 // LDS with coalesced HBM access; used when the real rows are dense)
 #pragma once
 
+#include <type_traits>
+
 #include "static_kernels_impl.hpp"
 
 namespace fcamd {
@@ -55,12 +57,16 @@ __device__ __forceinline__ void load_kernel_column(float2 (&w)[MAXE], int t, boo
   });
 }
 
-// phi / d for d = max(blur, eps) > 0: the hardware reciprocal (at most
-// 1 ulp) and one product, one rounding more than a correctly rounded
-// division and some ten instructions fewer per sample; a subnormal d
-// (only for a subnormal eps) keeps the division
+// phi / d for d = max(blur, eps) > 0.  With eps a normal number (the
+// launch-uniform case FAST) every d is one, and the hardware reciprocal (at
+// most 1 ulp) times phi replaces the IEEE division sequence: one rounding
+// more, some ten instructions fewer per sample.  A subnormal eps keeps the
+// division.
+constexpr float kMinNormal = 1.17549435e-38f;
+template <bool FAST>
 __device__ __forceinline__ float rl_div(float phi, float d) {
-  return d >= 1.17549435e-38f ? phi * __builtin_amdgcn_rcpf(d) : phi / d;
+  if constexpr (FAST) return phi * __builtin_amdgcn_rcpf(d);
+  return phi / d;
 }
 
 // column access at position pos through a base at position t
@@ -175,12 +181,19 @@ sk_xratio(RLXArgs a) {
   const float* fa = a.phi[v] + ra * a.rrow;
   const float* fb = a.phi[v] + rb * a.rrow;
   const float eps = a.eps;
-  sfft_for_outputs<L, true>(t, [&](int slot, int pos) {
-    float ya = 0.f, yb = 0.f;
-    if (oka) ya = rl_div(fa[pos], fmaxf(z[slot].x, eps));
-    if (okb) yb = rl_div(fb[pos], fmaxf(z[slot].y, eps));
-    z[slot] = make_float2(ya, yb);
-  });
+  auto ratio = [&](auto fast) {
+    sfft_for_outputs<L, true>(t, [&](int slot, int pos) {
+      float ya = 0.f, yb = 0.f;
+      constexpr bool F = decltype(fast)::value;
+      if (oka) ya = rl_div<F>(fa[pos], fmaxf(z[slot].x, eps));
+      if (okb) yb = rl_div<F>(fb[pos], fmaxf(z[slot].y, eps));
+      z[slot] = make_float2(ya, yb);
+    });
+  };
+  if (eps >= kMinNormal)
+    ratio(std::true_type{});
+  else
+    ratio(std::false_type{});
   sfft_run<L, false, -1>(z, t, row, addr, a.fft.stw);
   sfft_for_outputs<L, false>(t, [&](int slot, int pos) { row[addr(pos)] = z[slot]; });
   __syncthreads();
@@ -281,11 +294,18 @@ sk_xratio_flat(RLXArgs a) {
   xflat_load_rows<L, NT, MAXR>(a.phi[v] + r0 * L, st, nr);
   __syncthreads();
   const float eps = a.eps;
-  sfft_for_outputs<L, true>(t, [&](int slot, int pos) {
-    const float ya = oka ? rl_div(st[2 * c * SL + pos], fmaxf(z[slot].x, eps)) : 0.f;
-    const float yb = okb ? rl_div(st[(2 * c + 1) * SL + pos], fmaxf(z[slot].y, eps)) : 0.f;
-    z[slot] = make_float2(ya, yb);
-  });
+  auto ratio = [&](auto fast) {
+    constexpr bool F = decltype(fast)::value;
+    sfft_for_outputs<L, true>(t, [&](int slot, int pos) {
+      const float ya = oka ? rl_div<F>(st[2 * c * SL + pos], fmaxf(z[slot].x, eps)) : 0.f;
+      const float yb = okb ? rl_div<F>(st[(2 * c + 1) * SL + pos], fmaxf(z[slot].y, eps)) : 0.f;
+      z[slot] = make_float2(ya, yb);
+    });
+  };
+  if (eps >= kMinNormal)
+    ratio(std::true_type{});
+  else
+    ratio(std::false_type{});
   __syncthreads();
   sfft_run<L, false, -1>(z, t, row, addr, a.fft.stw);
   sfft_for_outputs<L, false>(t, [&](int slot, int pos) { row[addr(pos)] = z[slot]; });
