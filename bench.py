@@ -117,7 +117,7 @@ def config_timing(fc, torch, cfg, steps, checks, cpu_jobs, cpu_planes, cpu_singl
     lib = fc.lib()
     # selections of the lengths the tuner may change (-1: no compile-time kernel)
     keep = {(p, L): lib.fcamd_selected_static_variant(p, L)
-            for p, L in ((0, nx), (1, nx), (2, ny), (3, nz))}
+            for p, L in ((0, nx), (1, nx), (2, ny), (3, nz), (4, ny))}
     fc.autotune(img.shape, psf.shape, reps=3, stream=stream)
     variants = fc.selected_variants(img.shape)
     ms = timed_calls(torch, stream, lambda: fc.convolve3d(img, psf, out=out, workspace=ws,

@@ -180,7 +180,8 @@ def selected_variants(image_shape):
     """{pass name: variant name} currently selected for a C-order shape."""
     nz, ny, nx = image_shape
     out = {}
-    for p, (name, L) in enumerate([("x_forward", nx), ("x_inverse", nx), ("y", ny), ("z", nz)]):
+    for p, (name, L) in enumerate([("x_forward", nx), ("x_inverse", nx), ("y_forward", ny),
+                                   ("z", nz), ("y_inverse", ny)]):
         i = lib().fcamd_selected_static_variant(p, L)
         if i >= 0:
             out[name] = lib().fcamd_static_variant_name(p, L, i).decode()

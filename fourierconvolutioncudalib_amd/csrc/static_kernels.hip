@@ -177,7 +177,7 @@ bool static_ycol(const ConvShape& s, const FftDesc& fy, float2* S, int dir, cons
     a.k.center = s.cy;
     a.k_outer = s.kz;
   }
-  run_selected(e->col, e->selected[kSCol], a, dir, st);
+  run_selected(e->col, e->selected[dir < 0 ? kSCol : kSColInv], a, dir, st);
   return true;
 }
 

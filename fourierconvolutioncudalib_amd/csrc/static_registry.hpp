@@ -10,7 +10,9 @@
 
 namespace fcamd {
 
-enum StaticPass : int { kSxFwd = 0, kSxInv = 1, kSCol = 2, kSZ = 3, kSNumPasses = 4 };
+// kSCol is the forward y pass, kSColInv the inverse one; both choose among
+// the same column variants
+enum StaticPass : int { kSxFwd = 0, kSxInv = 1, kSCol = 2, kSZ = 3, kSColInv = 4, kSNumPasses = 5 };
 
 // Every launcher returns false when its variant does not apply to the
 // arguments (the dispatcher then uses variant 0).
@@ -53,14 +55,15 @@ struct StaticEntry {
   std::vector<Variant<ZFn>> zfused;
   XFwdFn xfwd_vpad = nullptr;  // x forward over a virtually padded image
   XInvFn xinv_crop = nullptr;  // x inverse writing the cropped interior
-  int selected[kSNumPasses] = {0, 0, 0, 0};
+  int selected[kSNumPasses] = {0, 0, 0, 0, 0};
   StaticBatch batch;
   StaticRL rl;
   int count(int pass) const {
     switch (pass) {
       case kSxFwd: return (int)xfwd.size();
       case kSxInv: return (int)xinv.size();
-      case kSCol: return (int)col.size();
+      case kSCol:
+      case kSColInv: return (int)col.size();
       case kSZ: return (int)zfused.size();
       default: return 0;
     }
@@ -70,7 +73,8 @@ struct StaticEntry {
     switch (pass) {
       case kSxFwd: return xfwd[i].name;
       case kSxInv: return xinv[i].name;
-      case kSCol: return col[i].name;
+      case kSCol:
+      case kSColInv: return col[i].name;
       default: return zfused[i].name;
     }
   }

@@ -138,7 +138,7 @@ int fcamd_autotune(const int* imDim, const int* kernelDim, int reps, void* strea
       // values finite (0x3c3c3c3c ~ 0.0115f)
       FC_HIP_CHECK(hipMemsetAsync(img, 0x3c, nimg, st));
       FC_HIP_CHECK(hipMemsetAsync(ker, 0x3c, nker, st));
-      const int axis_len[kSNumPasses] = {s.nx, s.nx, s.ny, s.nz};
+      const int axis_len[kSNumPasses] = {s.nx, s.nx, s.ny, s.nz, s.ny};
       const char* cache = std::getenv("FCAMD_TUNE_CACHE");
       // variants of one pass are timed in interleaved rounds (drift and
       // clock changes hit all of them alike) and ranked by their median;

@@ -215,7 +215,8 @@ FCAMD_API int fcamd_stream_copy_device(void* d_dst, const void* d_src, long long
 FCAMD_API int fcamd_radix_plan(int length, int* radices, int cap);
 
 /* 1 when a compile-time (register-resident) kernel serves this axis length
- * for the given pass (0 = x forward, 1 = x inverse, 2 = y, 3 = fused z). */
+ * for the given pass (0 = x forward, 1 = x inverse, 2 = y forward,
+ * 3 = fused z, 4 = y inverse). */
 FCAMD_API int fcamd_has_static_kernel(int pass, int length);
 
 /* Lists the lengths with compile-time kernels (up to `cap`); returns how many
@@ -223,7 +224,8 @@ FCAMD_API int fcamd_has_static_kernel(int pass, int length);
 FCAMD_API int fcamd_static_lengths(int* lengths, int cap);
 
 /* Variants of the compile-time kernels (pass: 0 x forward, 1 x inverse,
- * 2 y column, 3 fused z).  Names describe the tile shape and method; the
+ * 2 y column forward, 3 fused z, 4 y column inverse; 2 and 4 offer the same
+ * variants and are chosen separately).  Names describe the tile shape and method; the
  * selection is per (pass, length) and process-wide; fcamd_autotune sets it
  * from timings.  select returns 0 or an error code. */
 FCAMD_API int fcamd_static_variant_count(int pass, int length);

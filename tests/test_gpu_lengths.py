@@ -101,7 +101,7 @@ def test_correlate_static(fc, oracle):
 # kernel's z extent, so every (length, variant) runs at each extent a
 # workload can give it.
 KERNEL_EXTENTS = (3, 15, 21, 25, 31, 41, 51, 61)
-PASS_NAMES = {0: "x forward", 1: "x inverse", 2: "y", 3: "fused z"}
+PASS_NAMES = {0: "x forward", 1: "x inverse", 2: "y forward", 3: "fused z", 4: "y inverse"}
 
 
 def _variant_cases(p, L):
@@ -110,12 +110,12 @@ def _variant_cases(p, L):
     ext = [k for k in KERNEL_EXTENTS if k <= L]
     if p in (0, 1):
         return [((6, 10, L), (3, 5, k)) for k in (ext[0], ext[-1])]
-    if p == 2:
+    if p in (2, 4):
         return [((6, L, 24), (3, k, 5)) for k in (ext[0], ext[-1])]
     return [((L, 6, 18), (k, 5, 3)) for k in ext]
 
 
-@pytest.mark.parametrize("p", [0, 1, 2, 3])
+@pytest.mark.parametrize("p", [0, 1, 2, 3, 4])
 def test_every_variant(fc, oracle, p):
     """Every compile-time variant of pass p at every registered length,
     selected in turn, against the FP64 oracle: whatever the autotuner picks
@@ -155,7 +155,7 @@ def test_tuned_workloads_full_size(fc, oracle, cfg):
     img, psf = synth.workload(w)
     nz, ny, nx = img.shape
     keep = {(q, L): lib.fcamd_selected_static_variant(q, L)
-            for q, L in ((0, nx), (1, nx), (2, ny), (3, nz))}
+            for q, L in ((0, nx), (1, nx), (2, ny), (3, nz), (4, ny))}
     print(cfg, img.shape, psf.shape, flush=True)
     try:
         ti = torch.from_numpy(img).cuda()
@@ -224,13 +224,13 @@ def test_autotune_keeps_results_and_writes_cache(fc, oracle, tmp_path, monkeypat
     monkeypatch.setenv("FCAMD_TUNE_CACHE", str(cache))
     shape, kshape = (64, 96, 128), (9, 7, 5)
     lib = fc.lib()
-    tuned = [(0, 128), (1, 128), (2, 96), (3, 64)]
+    tuned = [(0, 128), (1, 128), (2, 96), (3, 64), (4, 96)]
     prev = {pl: lib.fcamd_selected_static_variant(*pl) for pl in tuned}
     fc.autotune(shape, kshape, reps=2)
     sel = fc.selected_variants(shape)
-    assert set(sel) == {"x_forward", "x_inverse", "y", "z"}
+    assert set(sel) == {"x_forward", "x_inverse", "y_forward", "z", "y_inverse"}
     lines = cache.read_text().splitlines()
-    assert len(lines) == 4 and all(len(l.split()) == 4 for l in lines)
+    assert len(lines) == 5 and all(len(l.split()) == 4 for l in lines)
     img = synth.uniform(shape, 17)
     k = synth.random_psf(kshape, 18)
     assert oracle.rel_l2(fc.convolve(img, k), oracle.numpy_fft_conv(img, k)) < 1e-5

@@ -25,7 +25,8 @@ def _noalloc(fc, out, img, k, op=0, stream=None):
 def test_noalloc_matches_device_entry_point(fc, oracle):
     import torch
     from fourierconvolutioncudalib_amd import synth
-    cases = [((64, 96, 128), (9, 7, 5), 0),   # compile-time lengths
+    cases = [((64, 128, 128), (15, 15, 15), 0),  # whole-plane x/y kernels
+             ((64, 96, 128), (9, 7, 5), 0),   # compile-time lengths
              ((40, 48, 64), (5, 5, 3), 1),    # smaller, correlation
              ((30, 22, 26), (5, 3, 3), 0),    # runtime lengths, smaller again
              ((12, 10, 14), (3, 3, 5), 0)]
