@@ -164,29 +164,34 @@ __host__ __device__ inline long column_tiles(int nbins, int TX, long nouter) {
   return (long)((nbins + TX - 1) / TX) * nouter;
 }
 
+// Tile indices, counts and rows stay below 2^31 (a grid dimension), so the
+// decode divides in 32 bits: it runs before a workgroup's first load, and a
+// 64-bit division is some hundred instructions.
 __host__ __device__ inline void column_tile_decode(int nbins, int TX, long nouter, long tile, int c,
                                                    long* o, int* x, bool* ok) {
-  const int nfull = nbins / TX, rem = nbins - nfull * TX;
+  const unsigned utx = (unsigned)TX, t32 = (unsigned)tile, rows = (unsigned)nouter;
+  const unsigned nfull = (unsigned)nbins / utx, rem = (unsigned)nbins - nfull * utx;
   if (rem == 1 && nfull > 0) {
     // the Nyquist tiles (strided single-bin gathers, slower than a full
     // tile) take the lowest indices so they run under the bulk of the grid
     // instead of trailing it
-    const long nnyq = (nouter + TX - 1) / TX;
-    if (tile < nnyq) {
-      *o = tile * TX + c;
+    const unsigned nnyq = (rows + utx - 1) / utx;
+    if (t32 < nnyq) {
+      const unsigned r = t32 * utx + (unsigned)c;
+      *o = (long)r;
       *x = nbins - 1;
-      *ok = *o < nouter;
+      *ok = r < rows;
     } else {
-      const long f = tile - nnyq;
-      *o = f / nfull;
-      *x = (int)(f - *o * nfull) * TX + c;
+      const unsigned f = t32 - nnyq, q = f / nfull;
+      *o = (long)q;
+      *x = (int)((f - q * nfull) * utx) + c;
       *ok = true;
     }
     return;
   }
-  const int ntx = (nbins + TX - 1) / TX;
-  *o = tile / ntx;
-  *x = (int)(tile - *o * ntx) * TX + c;
+  const unsigned ntx = ((unsigned)nbins + utx - 1) / utx, q = t32 / ntx;
+  *o = (long)q;
+  *x = (int)((t32 - q * ntx) * utx) + c;
   *ok = *x < nbins;
 }
 

@@ -857,6 +857,17 @@ __device__ __forceinline__ void zproduct(float2 (&v)[MAXE], const float2 (&w)[MA
   });
 }
 
+// the same product with the scale already in w (the tap path scales its
+// taps once as it loads them): v * (w.x, +-w.y)
+template <int L, int MAXE>
+__device__ __forceinline__ void zproduct_prescaled(float2 (&v)[MAXE], const float2 (&w)[MAXE], int t,
+                                                   bool cj) {
+  const float si = cj ? -1.f : 1.f;
+  sfft_for_outputs<L, false>(t, [&](int slot, int) {
+    v[slot] = cmul(v[slot], make_float2(w[slot].x, si * w[slot].y));
+  });
+}
+
 // taps per column the two-stage fast path can hold (its stage-1 radix)
 template <int L>
 constexpr int ktap_slots() {
@@ -884,7 +895,7 @@ __device__ __forceinline__ void zfused_sparse_body(const ZArgs& a, const ZTile<L
   auto addr = [&](int pos) { return pos * TX + c; };
   const int center = a.center, nvalid = a.nvalid;
   for (int j = t; j < nvalid; j += T)
-    ktaps[j * TX + c] = tile.ok ? tile.kcol[(long)j * a.k_plane] : make_float2(0.f, 0.f);
+    ktaps[j * TX + c] = tile.ok ? cscale(tile.kcol[(long)j * a.k_plane], a.scale) : make_float2(0.f, 0.f);
   // exp(+2 pi i b / R0) = conj(tw[b * R1]), tw[j] = exp(-2 pi i j / L)
   float2 e[NB1];
 #pragma unroll
@@ -922,7 +933,7 @@ __device__ __forceinline__ void zfused_sparse_body(const ZArgs& a, const ZTile<L
   }
   sfft_twiddle<L, false, -1, 1>(w, t, tw1);
   sfft_butterflies<L, false, -1, 1>(w, t);
-  zproduct<L>(v, w, t, a.conj != 0, a.scale);
+  zproduct_prescaled<L>(v, w, t, a.conj != 0);
   sfft_run<L, true, +1>(v, t, lds, addr, a.fft.stw);
   tile.template store<false>(a, t, v);
 }
