@@ -16,10 +16,15 @@ struct PlaneEntry {
 };
 
 #define FC_PLANE(NX, NY) {NX, NY, &plane_launch_fwd<NX, NY>, &plane_launch_inv<NX, NY>}
-// plane extents (nx, ny) with whole-plane kernels; each needs its
-// half-spectrum plane and its row exchange in one workgroup's LDS
-const PlaneEntry kPlanes[] = {FC_PLANE(64, 64), FC_PLANE(96, 96), FC_PLANE(128, 128),
-                              FC_PLANE(128, 64), FC_PLANE(64, 128)};
+#define FC_PLANE_ROW(NX)                                                          \
+  FC_PLANE(NX, 64), FC_PLANE(NX, 80), FC_PLANE(NX, 96), FC_PLANE(NX, 112),        \
+      FC_PLANE(NX, 120), FC_PLANE(NX, 128)
+// plane extents (nx, ny) with whole-plane kernels: every pair of the
+// compile-time lengths 64-128 with nx a multiple of 4 (35-139 KB of LDS:
+// the half-spectrum plane and the row exchange of one workgroup)
+const PlaneEntry kPlanes[] = {FC_PLANE_ROW(64),  FC_PLANE_ROW(80),  FC_PLANE_ROW(96),
+                              FC_PLANE_ROW(112), FC_PLANE_ROW(120), FC_PLANE_ROW(128)};
+#undef FC_PLANE_ROW
 #undef FC_PLANE
 
 // FCAMD_NO_PLANES=1 keeps the separate x and y passes

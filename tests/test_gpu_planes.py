@@ -25,7 +25,11 @@ CASES = [((64, 128, 128), (15, 15, 15)),   # the 128 x 128 x 64 workload
          ((17, 96, 96), (4, 6, 9)),         # even kernel extents (centre floor(k/2))
          ((12, 64, 128), (7, 13, 5)),
          ((20, 128, 64), (9, 3, 11)),
-         ((1, 128, 128), (1, 7, 5))]        # one plane
+         ((1, 128, 128), (1, 7, 5)),        # one plane
+         ((12, 120, 80), (5, 9, 11)),       # radix-5 and radix-15 row and column plans
+         ((9, 96, 112), (3, 7, 9)),
+         ((7, 64, 120), (7, 3, 21)),
+         ((30, 112, 64), (13, 11, 3))]
 
 
 def _ids(v):
@@ -78,5 +82,5 @@ def test_plane_path_runs(fc, oracle):
     print("plane path launches", plane, flush=True)
     assert plane["image_xfwd"] == 1 and plane["image_xinv"] == 1 and plane["zfused"] == 1
     assert plane["image_yfwd"] == 0 and plane["image_yinv"] == 0
-    other = passes((16, 80, 80), (5, 5, 5))
+    other = passes((16, 160, 160), (5, 5, 5))
     assert other["image_yfwd"] == 1 and other["image_yinv"] == 1
