@@ -619,6 +619,7 @@ void plane_forward(const ConvShape& s, const float* src, float2* S, const float*
                    float2* Kxy, hipStream_t st) {
   ensure_kernel_attributes();
   PlaneArgs a = plane_args(s, st);
+  if (!src) a.nz = 0;  // the kernel's planes only
   a.src = src;
   a.spec = S;
   a.kernel = kernel;

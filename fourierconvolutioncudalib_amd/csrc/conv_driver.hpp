@@ -124,6 +124,7 @@ void inverse_yx(const ConvShape& s, float2* S, float* dst, hipStream_t stream,
 // kernel rows) in one launch, inverse_yx in one launch.
 bool plane_path_available(const ConvShape& s);
 void plane_launch(const ConvShape& s, const PlaneArgs& a, int dir, hipStream_t stream);
+// src null: the kernel's planes only; kernel null: the image's planes only
 void plane_forward(const ConvShape& s, const float* src, float2* S, const float* kernel,
                    float2* Kxy, hipStream_t stream);
 void plane_inverse(const ConvShape& s, const float2* S, float* dst, hipStream_t stream);
