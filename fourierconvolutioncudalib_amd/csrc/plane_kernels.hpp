@@ -35,12 +35,15 @@ struct PlaneGeom {
   static constexpr int PAIRS = NY / 2;
   static constexpr int LPX = XRow<NX>::LP;
   static constexpr int want(int a, int b) { return a > b ? a : b; }
+  static constexpr int least(int a, int b) { return a < b ? a : b; }
   static constexpr int NT_ = (want(TX * PAIRS, TY * NB) + 63) / 64 * 64;
   static constexpr int NT = NT_ > 1024 ? 1024 : NT_;
-  static constexpr int XC = NT / TX < PAIRS ? NT / TX : PAIRS;  // row pairs at once
-  static constexpr int YC = NT / TY < NB ? NT / TY : NB;        // columns at once
-  static constexpr int P_FLOAT2 = NY * PB + 1;                  // + dummy slot
-  static constexpr int R_FLOAT2 = XC * LPX + 1;                 // + dummy slot
+  static constexpr int P_FLOAT2 = NY * PB + 1;  // + dummy slot
+  // row pairs at once: as many as the threads and the LDS left by P allow
+  static constexpr int XC_LDS = (160 * 1024 / 8 - P_FLOAT2 - 1) / LPX;
+  static constexpr int XC = least(least(NT / TX, PAIRS), XC_LDS);
+  static constexpr int YC = NT / TY < NB ? NT / TY : NB;  // columns at once
+  static constexpr int R_FLOAT2 = XC * LPX + 1;           // + dummy slot
   static constexpr long LDS_BYTES = 8L * (P_FLOAT2 + R_FLOAT2);
   static_assert(NX % 4 == 0 && NY % 2 == 0, "plane kernels take nx % 4 == 0, even ny");
   static_assert(XC >= 1 && YC >= 1, "plane too wide for one workgroup");

@@ -1,7 +1,8 @@
 """Whole-plane x/y kernels (plane_kernels.hpp): for plane extents with
 compile-time plane kernels the forward x-R2C and y-FFT run as one launch per
 z-plane (plus one per kernel plane) and the inverse y-IFFT and x-C2R as
-one, around the unchanged fused z pass.  Checked against the FP64 oracle at
+one, around the unchanged fused z pass (plane extents up to 256 x 64 and
+64 x 256 whose row pairs fit one exchange group).  Checked against the FP64 oracle at
 rel-L2 <= 1e-5 (the bar of every convolution here) for every instantiated
 plane extent, convolution and correlation, kernels from one tap to the full
 plane, off-centre and even extents, through the drop-in host entry point
@@ -29,7 +30,11 @@ CASES = [((64, 128, 128), (15, 15, 15)),   # the 128 x 128 x 64 workload
          ((12, 120, 80), (5, 9, 11)),       # radix-5 and radix-15 row and column plans
          ((9, 96, 112), (3, 7, 9)),
          ((7, 64, 120), (7, 3, 21)),
-         ((30, 112, 64), (13, 11, 3))]
+         ((30, 112, 64), (13, 11, 3)),
+         ((20, 256, 64), (9, 31, 5)),       # larger planes (one row-pair group)
+         ((11, 64, 256), (5, 7, 33)),
+         ((17, 96, 160), (7, 9, 15)),
+         ((13, 112, 144), (13, 5, 9))]
 
 
 def _ids(v):
