@@ -11,6 +11,7 @@
 // convolution) and re-applied by fcamd_autotune_load() (called on the first
 // tuning request).
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -145,6 +146,14 @@ int fcamd_autotune(const int* imDim, const int* kernelDim, int reps, void* strea
       // the current choice stays unless another is faster by more than
       // 0.5 %, the spread of repeated measurements
       constexpr int kRounds = 5;
+      // a timed block of at least ~1 ms of convolutions: below that (small
+      // volumes, tens of us per call) launch jitter outweighs the variants'
+      // differences
+      {
+        const float one = time_block(s, out, img, ker, ws, st, 3, e0, e1);
+        const int need = one > 0.f ? (int)std::ceil(1.0f / one) : reps;
+        reps = std::min(std::max(reps, need), 200);
+      }
       for (int pass = 0; pass < kSNumPasses; ++pass) {
         StaticEntry* e = static_lookup(axis_len[pass]);
         if (!e || e->count(pass) < 2) continue;
