@@ -262,9 +262,8 @@ BatchPlan::BatchPlan(int n, const int* imDims, const int* kernelDims, float* con
   // create every twiddle table now: nothing may allocate or copy while a
   // graph is being captured
   for (const Block& bl : blocks_) {
-    (void)fft_desc(bl.s.nx);
-    (void)fft_desc(bl.s.ny);
-    (void)fft_desc(bl.s.nz);
+    int lens[5];
+    for (int i = 0, n = table_lengths(bl.s, lens); i < n; ++i) (void)fft_desc(lens[i]);
   }
   if (nlanes > 0) {
     for (size_t i = 0; i + 1 < lanes_.size(); ++i) {

@@ -605,20 +605,41 @@ void pass_x_inverse(const ConvShape& s, const float2* S, float* dst, hipStream_t
 }
 
 namespace {
-PlaneArgs plane_args(const ConvShape& s, hipStream_t st) {
+PlaneArgs plane_args(const ConvShape& s, int dir, hipStream_t st) {
   PlaneArgs a{};
   a.fx = fft_desc(s.nx, st);
   a.fy = fft_desc(s.ny, st);
   a.nz = s.nz;
   a.nxp = padded_bins(s.nx);
+  if (plane_split(s, dir)) {
+    const FftDesc h = fft_desc(plane_half_length(s, dir), st);
+    FC_REQUIRE(h.stw != nullptr, kInternal, "half-length plan without a stage table");
+    (dir < 0 ? a.hx_stw : a.hy_stw) = h.stw;
+  }
   return a;
 }
 }  // namespace
 
+int table_lengths(const ConvShape& s, int* L) {
+  int n = 0;
+  auto add = [&](int len) {
+    for (int i = 0; i < n; ++i)
+      if (L[i] == len) return;
+    L[n++] = len;
+  };
+  add(s.nx);
+  add(s.ny);
+  add(s.nz);
+  if (plane_path_available(s))
+    for (int dir : {-1, +1})
+      if (plane_split(s, dir)) add(plane_half_length(s, dir));
+  return n;
+}
+
 void plane_forward(const ConvShape& s, const float* src, float2* S, const float* kernel,
                    float2* Kxy, hipStream_t st) {
   ensure_kernel_attributes();
-  PlaneArgs a = plane_args(s, st);
+  PlaneArgs a = plane_args(s, -1, st);
   if (!src) a.nz = 0;  // the kernel's planes only
   a.src = src;
   a.spec = S;
@@ -635,7 +656,7 @@ void plane_forward(const ConvShape& s, const float* src, float2* S, const float*
 
 void plane_inverse(const ConvShape& s, const float2* S, float* dst, hipStream_t st) {
   ensure_kernel_attributes();
-  PlaneArgs a = plane_args(s, st);
+  PlaneArgs a = plane_args(s, +1, st);
   a.spec = const_cast<float2*>(S);
   a.dst = dst;
   PassScope ps(kPassXInverse, st);

@@ -64,7 +64,7 @@ int stage_table_segments(int L, TwSegment* seg, int* total);
 // length is an internal error.  nullptr clears it.
 struct FftDescSet {
   int n = 0;
-  FftDesc d[3];
+  FftDesc d[5];  // table_lengths() of one shape
 };
 void set_fft_desc_override(const FftDescSet* set);
 
@@ -123,7 +123,16 @@ void inverse_yx(const ConvShape& s, float2* S, float* dst, hipStream_t stream,
 // fit one workgroup's LDS: forward_xy's image spectrum and kernel planes (no
 // kernel rows) in one launch, inverse_yx in one launch.
 bool plane_path_available(const ConvShape& s);
+// Whether direction dir (-1 forward, +1 inverse) runs the split kernels
+// (two workgroups per plane), which read the stage table of the half
+// length plane_half_length(s, dir) through PlaneArgs::hx_stw / hy_stw.
+bool plane_split(const ConvShape& s, int dir);
+int plane_half_length(const ConvShape& s, int dir);
 void plane_launch(const ConvShape& s, const PlaneArgs& a, int dir, hipStream_t stream);
+// Transform lengths whose tables a convolution of shape s reads (nx, ny, nz
+// and the split kernels' half lengths, without repeats); returns the count
+// (at most 5).
+int table_lengths(const ConvShape& s, int* L);
 // src null: the kernel's planes only; kernel null: the image's planes only
 void plane_forward(const ConvShape& s, const float* src, float2* S, const float* kernel,
                    float2* Kxy, hipStream_t stream);

@@ -152,6 +152,10 @@ struct PlaneArgs {
   float2* kplanes;      // kernel planes [kz][ny][nxp]
   int kx, ky, kz, cx, cy;
   float* dst;  // inverse: real planes [nz][ny][nx]
+  // split kernels: stage tables of the half-length plans (nx/2 forward,
+  // ny/2 inverse)
+  const float2* hx_stw;
+  const float2* hy_stw;
 };
 
 // Column tiles of TX consecutive bins over `nouter` rows.  When the bins end

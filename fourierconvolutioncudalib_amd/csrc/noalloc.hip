@@ -161,11 +161,10 @@ extern "C" int fcamd_convolve3d_device_noalloc(float* d_out, const float* d_imag
     FftDescSet set;
     char* tables = base + kWorkspaceMax;
     size_t used = 0;
-    for (int L : {s.nx, s.ny, s.nz}) {
-      bool have = false;
-      for (int i = 0; i < set.n; ++i) have = have || set.d[i].L == L;
-      if (have) continue;
-      used += prepare_length(L, tables + used, kTableBytes - used, &set.d[set.n], st);
+    int lens[5];
+    const int nlen = table_lengths(s, lens);
+    for (int i = 0; i < nlen; ++i) {
+      used += prepare_length(lens[i], tables + used, kTableBytes - used, &set.d[set.n], st);
       ++set.n;
     }
     OverrideScope scope(&set);

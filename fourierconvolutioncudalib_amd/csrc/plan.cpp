@@ -87,9 +87,8 @@ void* fcamd_conv_plan_create(const int* imDim, const int* kernelDim, const float
       p->Kx = reinterpret_cast<float2*>(base + wl.spectrum);
       p->Kxy = reinterpret_cast<float2*>(base + wl.spectrum + wl.krows);
       // twiddle tables now, so later executes allocate nothing
-      (void)fft_desc(s.nx);
-      (void)fft_desc(s.ny);
-      (void)fft_desc(s.nz);
+      int lens[5];
+      for (int i = 0, n = table_lengths(s, lens); i < n; ++i) (void)fft_desc(lens[i]);
       plan_kernel(*p, d_kernel, static_cast<hipStream_t>(stream));
     } catch (...) {
       if (p->own && p->ws) (void)hipFree(p->ws);

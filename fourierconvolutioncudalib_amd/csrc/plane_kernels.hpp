@@ -259,6 +259,317 @@ __global__ void __launch_bounds__((PlaneGeom<NX, NY>::NT)) sk_plane_inv(PlaneArg
   }
 }
 
+// ---------------------------------------------------------------------------
+// Split kernels: two workgroups per plane that share no work.
+//
+// One workgroup per plane puts a plane's x and y transforms on one CU, where
+// they are VALU-bound (about 0.6 M lane operations for a 128 x 128 plane,
+// ~8 us), and with fewer planes than CUs the other CUs idle.  Here each plane
+// takes two workgroups, each with half the transforms' work:
+//  - forward, part p computes the x bins k = p (mod 2).  A radix-2
+//    decimation in frequency turns the row pair's length-nx FFT into one of
+//    length h = nx/2 over u_p[n] = (z[n] + (-1)^p z[n + h]) W_nx^(p n),
+//    whose output m is bin 2m + p.  The two-real-rows split of bin k needs
+//    bin nx - k, of the same parity (nx % 4 == 0), so it stays in the part.
+//    Part p then transforms its columns and writes spectrum bins 2j + p.
+//  - inverse, part q computes the rows y = q (mod 2): outputs 2m + q of the
+//    length-ny column IFFT are the length-g = ny/2 IFFT of
+//    (X[k] + (-1)^q X[k + g]) W_ny^(-q k), formed as the plane is read; the
+//    x-C2R then runs on the part's g rows (ny % 4 == 0 keeps them in pairs).
+// The parts of a plane take blockIdx values equal mod 8, which the hardware
+// deals to one XCD, so the plane read twice and the interleaved bins written
+// by the two parts meet in that XCD's L2.
+// ---------------------------------------------------------------------------
+template <int S>
+__device__ __forceinline__ void plane_decode(int* q, int* part) {
+  const int b = blockIdx.x, r = b / 8;
+  *part = r % S;
+  *q = (r / S) * 8 + b % 8;
+}
+
+template <int S>
+inline int plane_blocks(int nplanes) {
+  return (nplanes + 7) / 8 * 8 * S;
+}
+
+template <int NX, int NY>
+struct PlaneGeomF2 {
+  static constexpr int H = NX / 2;                       // row FFT length
+  static constexpr int TX = threads_per_transform<H>();  // per row pair
+  static constexpr int TY = threads_per_transform<NY>();
+  static constexpr int NC = H / 2 + 1;  // columns of part 0 (part 1: H / 2)
+  static constexpr int PB = H + 2;      // plane row pitch (float2)
+  static constexpr int SL = 2 * PB;     // real staging pitch (floats)
+  static constexpr int PAIRS = NY / 2;
+  static constexpr int LPX = XRow<H>::LP;
+  static constexpr int want(int a, int b) { return a > b ? a : b; }
+  static constexpr int least(int a, int b) { return a < b ? a : b; }
+  static constexpr int NT_ = (want(TX * PAIRS, TY * NC) + 63) / 64 * 64;
+  static constexpr int NT = NT_ > 1024 ? 1024 : NT_;
+  static constexpr int P_FLOAT2 = NY * PB + 1;  // + dummy slot
+  static constexpr int XC_LDS = (160 * 1024 / 8 - P_FLOAT2 - 1) / LPX;
+  static constexpr int XC = least(least(NT / TX, PAIRS), XC_LDS);
+  static constexpr int YC = least(NT / TY, NC);
+  static constexpr int R_FLOAT2 = XC * LPX + 1;  // + dummy slot
+  static constexpr long LDS_BYTES = 8L * (P_FLOAT2 + R_FLOAT2);
+  static_assert(NX % 4 == 0 && NY % 2 == 0, "split forward takes nx % 4 == 0, even ny");
+  static_assert(XC >= 1 && YC >= 1, "plane too wide for one workgroup");
+  static_assert(LDS_BYTES <= 160 * 1024, "plane does not fit in LDS");
+};
+
+template <int NX, int NY>
+__global__ void __launch_bounds__((PlaneGeomF2<NX, NY>::NT)) sk_plane_fwd2(PlaneArgs a) {
+  using G = PlaneGeomF2<NX, NY>;
+  using PX = PlanOf<G::H, false>;
+  using PY = PlanOf<NY, false>;
+  __shared__ float2 P[G::P_FLOAT2];
+  __shared__ float2 rows[G::R_FLOAT2];
+  int q, p;
+  plane_decode<2>(&q, &p);
+  if (q >= a.nz + (a.kernel ? a.kz : 0)) return;  // workgroup-uniform, before any barrier
+  const bool image = q < a.nz;
+  const int plane = image ? q : q - a.nz;
+  const int c = threadIdx.x / G::TX, t0 = threadIdx.x % G::TX;
+  const bool lane = c < G::XC;
+  const int t = lane ? t0 : 0;
+  // W_nx^n of the thread's inputs n (part 1), requested before the plane
+  float2 w[PX::MAXE()];
+  sfft_for_inputs<G::H, false>(t, [&](int slot, int pos) { w[slot] = p ? a.fx.tw[pos] : make_float2(1.f, 0.f); });
+  float* st = reinterpret_cast<float*>(P);
+  if (image) {
+    const float* g = a.src + (long)plane * NX * NY;
+    if ((reinterpret_cast<uintptr_t>(g) & 15) == 0) {
+      constexpr int Q = NX / 4, N4 = Q * NY, IT = (N4 + G::NT - 1) / G::NT;
+      const float4* g4 = reinterpret_cast<const float4*>(g);
+      float4 r[IT];
+#pragma unroll
+      for (int j = 0; j < IT; ++j) {
+        const int i = threadIdx.x + j * G::NT;
+        r[j] = i < N4 ? g4[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+      }
+#pragma unroll
+      for (int j = 0; j < IT; ++j) {
+        const int i = threadIdx.x + j * G::NT;
+        if (i < N4) {
+          const int y = i / Q;
+          *reinterpret_cast<float4*>(st + y * G::SL + 4 * (i - y * Q)) = r[j];
+        }
+      }
+    } else {
+      for (int i = threadIdx.x; i < NX * NY; i += G::NT) {
+        const int y = i / NX;
+        st[y * G::SL + (i - y * NX)] = g[i];
+      }
+    }
+    __syncthreads();
+  }
+  const int ncols = G::NC - p;
+  for (int g0 = 0; g0 < G::PAIRS; g0 += G::XC) {
+    const int pr = g0 + c;
+    const bool act = lane && pr < G::PAIRS;
+    const int ya = 2 * pr, yb = ya + 1;
+    float2 v[PX::MAXE()];
+    if (image) {
+      sfft_for_inputs<G::H, false>(t, [&](int slot, int pos) {
+        float2 z0 = make_float2(0.f, 0.f), z1 = z0;
+        if (act) {
+          z0 = make_float2(st[ya * G::SL + pos], st[yb * G::SL + pos]);
+          z1 = make_float2(st[ya * G::SL + pos + G::H], st[yb * G::SL + pos + G::H]);
+        }
+        v[slot] = p ? cmul(csub(z0, z1), w[slot]) : cadd(z0, z1);
+      });
+    } else {
+      // row y of kernel plane `plane` holds kernel row (y + cy) mod NY, and
+      // its position x tap (x + cx) mod NX
+      int ia = ya + a.cy, ib = yb + a.cy;
+      if (ia >= NY) ia -= NY;
+      if (ib >= NY) ib -= NY;
+      const bool oka = act && ia < a.ky, okb = act && ib < a.ky;
+      const float* ka = a.kernel + ((long)plane * a.ky + (oka ? ia : 0)) * a.kx;
+      const float* kb = a.kernel + ((long)plane * a.ky + (okb ? ib : 0)) * a.kx;
+      const int cx = a.cx, kx = a.kx;
+      auto tap = [&](int pos) {
+        int x = pos + cx;
+        if (x >= NX) x -= NX;
+        float xa = 0.f, xb = 0.f;
+        if (x < kx) {
+          if (oka) xa = ka[x];
+          if (okb) xb = kb[x];
+        }
+        return make_float2(xa, xb);
+      };
+      sfft_for_inputs<G::H, false>(t, [&](int slot, int pos) {
+        const float2 z0 = tap(pos), z1 = tap(pos + G::H);
+        v[slot] = p ? cmul(csub(z0, z1), w[slot]) : cadd(z0, z1);
+      });
+    }
+    auto addr = [&](int pos) { return act ? c * G::LPX + XRow<G::H>::at(pos) : G::R_FLOAT2 - 1; };
+    sfft_run<G::H, false, -1>(v, t, rows, addr, a.hx_stw);
+    sfft_for_outputs<G::H, false>(t, [&](int slot, int pos) { rows[addr(pos)] = v[slot]; });
+    __syncthreads();
+    if (act) {
+      // local column j is bin k = 2j + p; bin nx - k is output jm
+      const float2* row = rows + c * G::LPX;
+      for (int j = t; j < ncols; j += G::TX) {
+        const int jm = p ? G::H - 1 - j : (j == 0 ? 0 : G::H - j);
+        const float2 Z = row[XRow<G::H>::at(j)];
+        const float2 Zm = row[XRow<G::H>::at(jm)];
+        P[ya * G::PB + j] = make_float2(0.5f * (Z.x + Zm.x), 0.5f * (Z.y - Zm.y));
+        P[yb * G::PB + j] = make_float2(0.5f * (Z.y + Zm.y), 0.5f * (Zm.x - Z.x));
+      }
+    }
+    __syncthreads();  // the exchange rows are reused by the next group
+  }
+  // columns of the part: FFT along y, bins 2j + p of the spectrum plane
+  float2* out = (image ? a.spec : a.kplanes) + (long)plane * NY * a.nxp + p;
+  const long nxp = a.nxp;
+  const int cc = threadIdx.x % G::YC, u0 = threadIdx.x / G::YC;
+  const bool ylane = u0 < G::TY;
+  const int u = ylane ? u0 : 0;
+  for (int g0 = 0; g0 < ncols; g0 += G::YC) {
+    const int col = g0 + cc;
+    const bool act = ylane && col < ncols;
+    auto addr = [&](int pos) { return act ? pos * G::PB + col : G::P_FLOAT2 - 1; };
+    float2 v[PY::MAXE()];
+    sfft_for_inputs<NY, false>(u, [&](int slot, int pos) { v[slot] = P[addr(pos)]; });
+    __syncthreads();  // every input of the column is read before the exchange
+    sfft_run<NY, false, -1>(v, u, P, addr, a.fy.stw);
+    if (act)
+      sfft_for_outputs<NY, false>(u, [&](int slot, int pos) { out[(long)pos * nxp + 2 * col] = v[slot]; });
+  }
+}
+
+template <int NX, int NY>
+struct PlaneGeomI2 {
+  static constexpr int GY = NY / 2;                       // column IFFT length
+  static constexpr int TX = threads_per_transform<NX>();  // per row pair
+  static constexpr int TY = threads_per_transform<GY>();
+  static constexpr int NB = NX / 2 + 1;
+  static constexpr int PB = NB + 1;
+  static constexpr int SL = 2 * PB;
+  static constexpr int PAIRS = GY / 2;
+  static constexpr int LPX = XRow<NX>::LP;
+  static constexpr int want(int a, int b) { return a > b ? a : b; }
+  static constexpr int least(int a, int b) { return a < b ? a : b; }
+  static constexpr int NT_ = (want(TX * PAIRS, TY * NB) + 63) / 64 * 64;
+  static constexpr int NT = NT_ > 1024 ? 1024 : NT_;
+  static constexpr int P_FLOAT2 = GY * PB + 1;  // + dummy slot
+  static constexpr int XC_LDS = (160 * 1024 / 8 - P_FLOAT2 - 1) / LPX;
+  static constexpr int XC = least(least(NT / TX, PAIRS), XC_LDS);
+  static constexpr int YC = least(NT / TY, NB);
+  static constexpr int R_FLOAT2 = XC * LPX + 1;  // + dummy slot
+  static constexpr long LDS_BYTES = 8L * (P_FLOAT2 + R_FLOAT2);
+  static_assert(NX % 4 == 0 && NY % 4 == 0, "split inverse takes nx % 4 == 0, ny % 4 == 0");
+  static_assert(XC >= 1 && YC >= 1, "plane too wide for one workgroup");
+  static_assert(LDS_BYTES <= 160 * 1024, "plane does not fit in LDS");
+};
+
+template <int NX, int NY>
+__global__ void __launch_bounds__((PlaneGeomI2<NX, NY>::NT)) sk_plane_inv2(PlaneArgs a) {
+  using G = PlaneGeomI2<NX, NY>;
+  using PX = PlanOf<NX, false>;
+  using PY = PlanOf<G::GY, false>;
+  __shared__ float2 P[G::P_FLOAT2];
+  __shared__ float2 rows[G::R_FLOAT2];
+  int plane, q;
+  plane_decode<2>(&plane, &q);
+  if (plane >= a.nz) return;  // workgroup-uniform, before any barrier
+  {
+    // row k of P: X[k] + X[k + g] (part 0) or (X[k] - X[k + g]) W_ny^-k
+    const float2* sp = a.spec + (long)plane * NY * a.nxp;
+    const long half = (long)G::GY * a.nxp;
+    constexpr int N = G::GY * G::NB, IT = (N + G::NT - 1) / G::NT;
+    float2 r0[IT], r1[IT], w[IT];
+#pragma unroll
+    for (int j = 0; j < IT; ++j) {
+      const int i = threadIdx.x + j * G::NT;
+      const int k = i / G::NB;
+      const long o = (long)k * a.nxp + (i - k * G::NB);
+      r0[j] = i < N ? sp[o] : make_float2(0.f, 0.f);
+      r1[j] = i < N ? sp[o + half] : make_float2(0.f, 0.f);
+      w[j] = q && i < N ? a.fy.tw[k] : make_float2(1.f, 0.f);
+    }
+#pragma unroll
+    for (int j = 0; j < IT; ++j) {
+      const int i = threadIdx.x + j * G::NT;
+      const int k = i / G::NB;
+      if (i < N) P[k * G::PB + (i - k * G::NB)] = q ? cmulc(csub(r0[j], r1[j]), w[j]) : cadd(r0[j], r1[j]);
+    }
+  }
+  __syncthreads();
+  {
+    // the columns' length-g IFFTs, in place
+    const int cc = threadIdx.x % G::YC, u0 = threadIdx.x / G::YC;
+    const bool ylane = u0 < G::TY;
+    const int u = ylane ? u0 : 0;
+    for (int g0 = 0; g0 < G::NB; g0 += G::YC) {
+      const int col = g0 + cc;
+      const bool act = ylane && col < G::NB;
+      auto addr = [&](int pos) { return act ? pos * G::PB + col : G::P_FLOAT2 - 1; };
+      float2 v[PY::MAXE()];
+      sfft_for_inputs<G::GY, false>(u, [&](int slot, int pos) { v[slot] = P[addr(pos)]; });
+      __syncthreads();  // every input of the column is read before the exchange
+      sfft_run<G::GY, false, +1>(v, u, P, addr, a.hy_stw);
+      sfft_for_outputs<G::GY, false>(u, [&](int slot, int pos) { P[addr(pos)] = v[slot]; });
+    }
+  }
+  __syncthreads();
+  // x-C2R of the part's row pairs; the real rows go to the staging pitch
+  // inside their own spectrum rows
+  float* st = reinterpret_cast<float*>(P);
+  const int c = threadIdx.x / G::TX, t0 = threadIdx.x % G::TX;
+  const bool lane = c < G::XC;
+  const int t = lane ? t0 : 0;
+  for (int g0 = 0; g0 < G::PAIRS; g0 += G::XC) {
+    const int pr = g0 + c;
+    const bool act = lane && pr < G::PAIRS;
+    const int ya = 2 * pr, yb = ya + 1;
+    float2 v[PX::MAXE()];
+    sfft_for_inputs<NX, false>(t, [&](int slot, int pos) {
+      const bool mirror = pos >= G::NB;
+      const int kb = mirror ? NX - pos : pos;
+      float2 A = act ? P[ya * G::PB + kb] : make_float2(0.f, 0.f);
+      float2 B = act ? P[yb * G::PB + kb] : make_float2(0.f, 0.f);
+      if (kb == 0 || 2 * kb == NX) {
+        A.y = 0.f;
+        B.y = 0.f;
+      }
+      if (mirror) {
+        A.y = -A.y;
+        B.y = -B.y;
+      }
+      v[slot] = make_float2(A.x - B.y, A.y + B.x);
+    });
+    auto addr = [&](int pos) { return act ? c * G::LPX + XRow<NX>::at(pos) : G::R_FLOAT2 - 1; };
+    sfft_run<NX, false, +1>(v, t, rows, addr, a.fx.stw);
+    if (act)
+      sfft_for_outputs<NX, false>(t, [&](int slot, int pos) {
+        st[ya * G::SL + pos] = v[slot].x;
+        st[yb * G::SL + pos] = v[slot].y;
+      });
+    __syncthreads();  // the exchange rows are reused by the next group
+  }
+  // local row m is image row 2m + q
+  float* g = a.dst + (long)plane * NX * NY + (long)q * NX;
+  if ((reinterpret_cast<uintptr_t>(g) & 15) == 0) {
+    constexpr int Q = NX / 4, N4 = Q * G::GY, IT = (N4 + G::NT - 1) / G::NT;
+#pragma unroll
+    for (int j = 0; j < IT; ++j) {
+      const int i = threadIdx.x + j * G::NT;
+      if (i < N4) {
+        const int m = i / Q, x4 = i - m * Q;
+        *reinterpret_cast<float4*>(g + (long)2 * m * NX + 4 * x4) =
+            *reinterpret_cast<const float4*>(st + m * G::SL + 4 * x4);
+      }
+    }
+  } else {
+    for (int i = threadIdx.x; i < NX * G::GY; i += G::NT) {
+      const int m = i / NX, x = i - m * NX;
+      g[(long)2 * m * NX + x] = st[m * G::SL + x];
+    }
+  }
+}
+
 template <int NX, int NY>
 void plane_launch_fwd(const PlaneArgs& a, hipStream_t st) {
   using G = PlaneGeom<NX, NY>;
@@ -270,6 +581,36 @@ template <int NX, int NY>
 void plane_launch_inv(const PlaneArgs& a, hipStream_t st) {
   using G = PlaneGeom<NX, NY>;
   hipLaunchKernelGGL((sk_plane_inv<NX, NY>), dim3((unsigned)a.nz), dim3(G::NT), 0, st, a);
+}
+
+// the split kernels need compile-time plans of the half lengths; the
+// forward split measured neutral or slower for planes below 10240 points
+// (nx x ny x nz, GPU time per convolution: 64 x 64 x 64 15.19 -> 15.12 us,
+// 96 x 96 x 64 22.59 -> 23.44 us, 64 x 128 x 64 19.41 -> 19.24 us) and
+// faster from there (128 x 80 x 64: 23.27 -> 22.26 us, 128 x 128 x 64:
+// 28.58 -> 27.10 us, 256 x 64 x 32: 26.05 -> 24.44 us, 64 x 256 x 32:
+// 23.95 -> 22.79 us)
+template <int NX, int NY>
+constexpr bool plane_fwd2_ok() {
+  return NX * NY >= 10240 && static_plan(NX / 2).n > 0;
+}
+template <int NX, int NY>
+constexpr bool plane_inv2_ok() {
+  return NY % 4 == 0 && static_plan(NY / 2).n > 0;
+}
+
+template <int NX, int NY>
+void plane_launch_fwd2(const PlaneArgs& a, hipStream_t st) {
+  using G = PlaneGeomF2<NX, NY>;
+  const int nblk = plane_blocks<2>(a.nz + (a.kernel ? a.kz : 0));
+  hipLaunchKernelGGL((sk_plane_fwd2<NX, NY>), dim3((unsigned)nblk), dim3(G::NT), 0, st, a);
+}
+
+template <int NX, int NY>
+void plane_launch_inv2(const PlaneArgs& a, hipStream_t st) {
+  using G = PlaneGeomI2<NX, NY>;
+  hipLaunchKernelGGL((sk_plane_inv2<NX, NY>), dim3((unsigned)plane_blocks<2>(a.nz)), dim3(G::NT),
+                     0, st, a);
 }
 
 }  // namespace fcamd

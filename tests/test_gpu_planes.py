@@ -2,7 +2,10 @@
 compile-time plane kernels the forward x-R2C and y-FFT run as one launch per
 z-plane (plus one per kernel plane) and the inverse y-IFFT and x-C2R as
 one, around the unchanged fused z pass (plane extents up to 256 x 64 and
-64 x 256 whose row pairs fit one exchange group).  Checked against the FP64 oracle at
+64 x 256 whose row pairs fit one exchange group).  Volumes with few planes
+take the split kernels (two workgroups per plane: even/odd x bins forward,
+even/odd rows inverse); the cases cover split and whole-plane kernels in
+either direction.  Checked against the FP64 oracle at
 rel-L2 <= 1e-5 (the bar of every convolution here) for every instantiated
 plane extent, convolution and correlation, kernels from one tap to the full
 plane, off-centre and even extents, through the drop-in host entry point
@@ -34,7 +37,11 @@ CASES = [((64, 128, 128), (15, 15, 15)),   # the 128 x 128 x 64 workload
          ((20, 256, 64), (9, 31, 5)),       # larger planes (one row-pair group)
          ((11, 64, 256), (5, 7, 33)),
          ((17, 96, 160), (7, 9, 15)),
-         ((13, 112, 144), (13, 5, 9))]
+         ((13, 112, 144), (13, 5, 9)),
+         # 128 x 128 planes under the split rule (256 CUs): forward and
+         # inverse one workgroup per plane; forward whole, inverse split
+         ((200, 128, 128), (5, 5, 5)),
+         ((100, 128, 128), (31, 5, 5))]
 
 
 def _ids(v):
