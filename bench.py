@@ -100,6 +100,10 @@ def timed_calls(torch, stream, fn, reps, warm=2):
     return e0.elapsed_time(e1) / reps
 
 
+# variants are tuned per workload unless --tuned applies recorded choices
+TUNE = True
+
+
 def config_timing(fc, torch, cfg, steps, checks, cpu_jobs, cpu_planes, cpu_single):
     """A further workload timed like the metric: device-resident, autotuned
     variants (restored afterwards), HIP events around `steps` calls; its
@@ -118,7 +122,8 @@ def config_timing(fc, torch, cfg, steps, checks, cpu_jobs, cpu_planes, cpu_singl
     # selections of the lengths the tuner may change (-1: no compile-time kernel)
     keep = {(p, L): lib.fcamd_selected_static_variant(p, L)
             for p, L in ((0, nx), (1, nx), (2, ny), (3, nz), (4, ny))}
-    fc.autotune(img.shape, psf.shape, reps=3, stream=stream)
+    if TUNE:
+        fc.autotune(img.shape, psf.shape, reps=3, stream=stream)
     variants = fc.selected_variants(img.shape)
     ms = timed_calls(torch, stream, lambda: fc.convolve3d(img, psf, out=out, workspace=ws,
                                                             stream=stream), steps, warm=3)
@@ -467,6 +472,10 @@ def main():
                     help="skip the CPU leg (CPU baseline and the FP64 parity checks)")
     ap.add_argument("--no-batched", action="store_true", help="time the metric workload only")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--tuned", metavar="FILE",
+                    help="no tuning: apply the variant choices recorded in FILE by an earlier "
+                         "run with FCAMD_TUNE_CACHE=FILE (so that a profiled run uses the "
+                         "kernels an unprofiled one chose)")
     ap.add_argument("--only", choices=["batched", "rl", "adjoint", "configs", "primes"],
                     help="run one side measurement alone and print its JSON (for profiling)")
     args = ap.parse_args()
@@ -474,6 +483,11 @@ def main():
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               "(every convolution here runs on a single device)", file=sys.stderr)
         sys.exit(2)
+    if args.tuned:
+        import fourierconvolutioncudalib_amd as fc
+        global TUNE
+        TUNE = False
+        fc.check(fc.lib().fcamd_autotune_load(args.tuned.encode()))
     if args.only:
         import torch
         import fourierconvolutioncudalib_amd as fc
@@ -513,7 +527,7 @@ def main():
         fc.convolve3d(img, psf, out=out, workspace=ws, stream=stream)
 
     # plan-time tuning of the kernel variants for this shape (not timed)
-    if not args.no_autotune:
+    if TUNE and not args.no_autotune:
         fc.autotune(img.shape, psf.shape, reps=5, stream=stream)
     variants = fc.selected_variants(img.shape)
 
