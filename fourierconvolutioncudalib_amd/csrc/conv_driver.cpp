@@ -630,9 +630,8 @@ int table_lengths(const ConvShape& s, int* L) {
   add(s.nx);
   add(s.ny);
   add(s.nz);
-  if (plane_path_available(s))
-    for (int dir : {-1, +1})
-      if (plane_split(s, dir)) add(plane_half_length(s, dir));
+  for (int dir : {-1, +1})
+    if (plane_split(s, dir)) add(plane_half_length(s, dir));
   return n;
 }
 
@@ -738,15 +737,15 @@ void enqueue_convolution(const ConvShape& s, float* d_out, const float* d_image,
   float2* S = reinterpret_cast<float2*>(base);
   float2* Kx = reinterpret_cast<float2*>(base + wl.spectrum);
   float2* Kxy = reinterpret_cast<float2*>(base + wl.spectrum + wl.krows);
-  if (plane_path_available(s)) {
+  if (plane_path(s, -1))
     plane_forward(s, d_image, S, d_kernel, Kxy, st);
-    z_pass(s, S, Kxy, false, op == Op::kCorrelate, st);
-    plane_inverse(s, S, d_out, st);
-  } else {
+  else
     forward_xy(s, d_image, S, d_kernel, Kx, Kxy, st);
-    z_pass(s, S, Kxy, false, op == Op::kCorrelate, st);
+  z_pass(s, S, Kxy, false, op == Op::kCorrelate, st);
+  if (plane_path(s, +1))
+    plane_inverse(s, S, d_out, st);
+  else
     inverse_yx(s, S, d_out, st);
-  }
   FC_HIP_CHECK(hipGetLastError());
 }
 

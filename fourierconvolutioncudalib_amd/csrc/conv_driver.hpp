@@ -119,13 +119,15 @@ void z_pass(const ConvShape& s, float2* S, const float2* operand, bool dense, bo
 void inverse_yx(const ConvShape& s, float2* S, float* dst, hipStream_t stream,
                 const VPad* vp = nullptr);
 
-// Whole-plane x and y transforms (plane_kernels.hip) for plane extents that
-// fit one workgroup's LDS: forward_xy's image spectrum and kernel planes (no
-// kernel rows) in one launch, inverse_yx in one launch.
-bool plane_path_available(const ConvShape& s);
-// Whether direction dir (-1 forward, +1 inverse) runs the split kernels
-// (two workgroups per plane), which read the stage table of the half
-// length plane_half_length(s, dir) through PlaneArgs::hx_stw / hy_stw.
+// Whole-plane x and y transforms (plane_kernels.hip) for plane extents whose
+// half-spectrum plane, or half of it, fits one workgroup's LDS: forward_xy's
+// image spectrum and kernel planes (no kernel rows) in one launch,
+// inverse_yx in one launch; each direction is chosen on its own.
+// Whether direction dir (-1 forward, +1 inverse) runs on plane kernels.
+bool plane_path(const ConvShape& s, int dir);
+// Whether it runs the split kernels (two workgroups per plane), which read
+// the stage table of the half length plane_half_length(s, dir) through
+// PlaneArgs::hx_stw / hy_stw.
 bool plane_split(const ConvShape& s, int dir);
 int plane_half_length(const ConvShape& s, int dir);
 void plane_launch(const ConvShape& s, const PlaneArgs& a, int dir, hipStream_t stream);

@@ -27,7 +27,7 @@ void plan_kernel(ConvPlan& p, const float* d_kernel, hipStream_t st) {
   // the kernel's rows and planes only (no image in these launches), by the
   // same kernels a plain call of this shape uses, so that executing the
   // plan reproduces the plain call bit for bit
-  if (plane_path_available(p.s)) {
+  if (plane_path(p.s, -1)) {
     plane_forward(p.s, nullptr, nullptr, d_kernel, p.Kxy, st);
   } else {
     pass_x_forward(p.s, nullptr, nullptr, d_kernel, p.Kx, st);
@@ -113,15 +113,15 @@ int fcamd_conv_plan_execute(void* plan, float* d_out, const float* d_image, void
     FC_REQUIRE(d_out && d_image, kInvalidArgument, "null device pointer");
     ConvPlan& p = *static_cast<ConvPlan*>(plan);
     hipStream_t st = static_cast<hipStream_t>(stream);
-    if (plane_path_available(p.s)) {
+    if (plane_path(p.s, -1))
       plane_forward(p.s, d_image, p.S, nullptr, nullptr, st);
-      z_pass(p.s, p.S, p.Kxy, false, p.op == Op::kCorrelate, st);
-      plane_inverse(p.s, p.S, d_out, st);
-    } else {
+    else
       forward_xy(p.s, d_image, p.S, nullptr, nullptr, nullptr, st);
-      z_pass(p.s, p.S, p.Kxy, false, p.op == Op::kCorrelate, st);
+    z_pass(p.s, p.S, p.Kxy, false, p.op == Op::kCorrelate, st);
+    if (plane_path(p.s, +1))
+      plane_inverse(p.s, p.S, d_out, st);
+    else
       inverse_yx(p.s, p.S, d_out, st);
-    }
     FC_HIP_CHECK(hipGetLastError());
   });
 }

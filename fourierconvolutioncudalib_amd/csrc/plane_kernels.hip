@@ -11,10 +11,13 @@ namespace fcamd {
 namespace {
 
 using PlaneFn = void (*)(const PlaneArgs&, hipStream_t);
+using OccFn = int (*)();
 struct PlaneEntry {
   int nx, ny;
-  PlaneFn fwd, inv;
+  PlaneFn fwd, inv;    // one workgroup per plane (null for wide planes)
   PlaneFn fwd2, inv2;  // split kernels, where the half lengths have plans
+  bool wide;           // only the wide split kernels
+  OccFn occ_fwd2, occ_inv2;  // wide: workgroups per CU of fwd2 / inv2
 };
 
 template <int NX, int NY>
@@ -34,7 +37,12 @@ constexpr PlaneFn split_inv() {
 
 #define FC_PLANE(NX, NY)                                                                 \
   {NX, NY, &plane_launch_fwd<NX, NY>, &plane_launch_inv<NX, NY>, split_fwd<NX, NY>(), \
-   split_inv<NX, NY>()}
+   split_inv<NX, NY>(), false, nullptr, nullptr}
+#define FC_PLANE_WIDE(NX, NY)                                                                      \
+  {NX, NY, nullptr, nullptr, &plane_launch_fwd2w<NX, NY>, &plane_launch_inv2w<NX, NY>, true, \
+   &plane_occupancy_fwd2w<NX, NY>, &plane_occupancy_inv2w<NX, NY>}
+#define FC_PLANE_WIDE_INV(NX, NY) \
+  {NX, NY, nullptr, nullptr, nullptr, &plane_launch_inv2w<NX, NY>, true, nullptr, &plane_occupancy_inv2w<NX, NY>}
 #define FC_PLANE_ROW(NX)                                                          \
   FC_PLANE(NX, 64), FC_PLANE(NX, 80), FC_PLANE(NX, 96), FC_PLANE(NX, 112),        \
       FC_PLANE(NX, 120), FC_PLANE(NX, 128)
@@ -57,7 +65,21 @@ const PlaneEntry kPlanes[] = {
     FC_PLANE(160, 96), FC_PLANE(160, 112), FC_PLANE(192, 64), FC_PLANE(192, 80),
     FC_PLANE(192, 96), FC_PLANE(224, 64), FC_PLANE(224, 80), FC_PLANE(240, 64),
     FC_PLANE(256, 64),
+    // wide planes: half a half-spectrum plane per workgroup (35-150 KB).
+    // 240 x 240 keeps the separate forward passes: its wide forward
+    // measured slower at 96 planes (57.8 -> 61.4 us per convolution),
+    // where the other wide planes gained at 64 planes (256 x 240: 72.1 ->
+    // 64.5 us forward split, 55.8 us both).
+    FC_PLANE_WIDE(128, 160), FC_PLANE_WIDE(160, 128), FC_PLANE_WIDE(144, 144),
+    FC_PLANE_WIDE(160, 160), FC_PLANE_WIDE(128, 192), FC_PLANE_WIDE(192, 128),
+    FC_PLANE_WIDE(160, 192), FC_PLANE_WIDE(192, 160), FC_PLANE_WIDE(192, 192),
+    FC_PLANE_WIDE(224, 224), FC_PLANE_WIDE_INV(240, 240), FC_PLANE_WIDE(256, 256),
+    FC_PLANE_WIDE(128, 256), FC_PLANE_WIDE(256, 128), FC_PLANE_WIDE(192, 256),
+    FC_PLANE_WIDE(256, 192), FC_PLANE_WIDE(224, 256), FC_PLANE_WIDE(256, 224),
+    FC_PLANE_WIDE(240, 256), FC_PLANE_WIDE(256, 240),
 };
+#undef FC_PLANE_WIDE_INV
+#undef FC_PLANE_WIDE
 #undef FC_PLANE_ROW
 #undef FC_PLANE
 
@@ -118,30 +140,45 @@ bool split_auto(const ConvShape& s, int dir) {
   return 2 * planes <= cu_count();
 }
 
-}  // namespace
-
-bool plane_path_available(const ConvShape& s) {
-  // the per-stage tables of the compile-time plans carry the twiddles
-  return plane_entry(s) != nullptr && static_lookup(s.nx) && static_lookup(s.ny);
+// Wide planes replace two passes through HBM rather than one workgroup per
+// plane.  Measured on MI355X (GPU time per convolution, separate passes
+// against the wide kernel in one direction): the forward gains while its
+// workgroups fit one round (256 x 256 x 64, 15^3: 64.6 -> 61.3 us) and
+// loses at two (256 x 256 x 128, 21^3, 298 workgroups of one per CU: 86.9
+// -> 96.1 us); the inverse gains at one round (86.9 -> 77.7 us) and at two
+// (256 x 256 x 256: 133.0 -> 124.9 us).
+bool split_auto_wide(const PlaneEntry& e, const ConvShape& s, int dir) {
+  if (dir < 0) return 2 * (s.nz + s.kz) <= cu_count() * e.occ_fwd2();
+  return 2 * s.nz <= 2 * cu_count() * e.occ_inv2();
 }
+
+}  // namespace
 
 bool plane_split(const ConvShape& s, int dir) {
   const PlaneEntry* e = plane_entry(s);
   if (e == nullptr || (dir < 0 ? e->fwd2 : e->inv2) == nullptr) return false;
+  // the per-stage tables of the compile-time plans carry the twiddles
+  if (!static_lookup(s.nx) || !static_lookup(s.ny)) return false;
   switch (split_mode()) {
     case 0: return false;
     case 1: return dir < 0;
     case 2: return dir > 0;
     case 3: return true;
-    default: return split_auto(s, dir);
+    default: return e->wide ? split_auto_wide(*e, s, dir) : split_auto(s, dir);
   }
+}
+
+bool plane_path(const ConvShape& s, int dir) {
+  const PlaneEntry* e = plane_entry(s);
+  if (e == nullptr || !static_lookup(s.nx) || !static_lookup(s.ny)) return false;
+  return (dir < 0 ? e->fwd : e->inv) != nullptr || plane_split(s, dir);
 }
 
 int plane_half_length(const ConvShape& s, int dir) { return dir < 0 ? s.nx / 2 : s.ny / 2; }
 
 void plane_launch(const ConvShape& s, const PlaneArgs& a, int dir, hipStream_t st) {
   const PlaneEntry* e = plane_entry(s);
-  FC_REQUIRE(e != nullptr, kInternal, "no whole-plane kernels for this plane extent");
+  FC_REQUIRE(plane_path(s, dir), kInternal, "no whole-plane kernels for this plane extent");
   if (plane_split(s, dir)) {
     FC_REQUIRE((dir < 0 ? a.hx_stw : a.hy_stw) != nullptr, kInternal,
                "split plane kernels without the half-length table");

@@ -2,7 +2,8 @@
 compile-time plane kernels the forward x-R2C and y-FFT run as one launch per
 z-plane (plus one per kernel plane) and the inverse y-IFFT and x-C2R as
 one, around the unchanged fused z pass (plane extents up to 256 x 64 and
-64 x 256 whose row pairs fit one exchange group).  Volumes with few planes
+64 x 256 whose row pairs fit one exchange group, and wide planes up to
+256 x 256 whose even or odd half fits).  Volumes with few planes
 take the split kernels (two workgroups per plane: even/odd x bins forward,
 even/odd rows inverse); the cases cover split and whole-plane kernels in
 either direction.  Checked against the FP64 oracle at
@@ -41,7 +42,17 @@ CASES = [((64, 128, 128), (15, 15, 15)),   # the 128 x 128 x 64 workload
          # 128 x 128 planes under the split rule (256 CUs): forward and
          # inverse one workgroup per plane; forward whole, inverse split
          ((200, 128, 128), (5, 5, 5)),
-         ((100, 128, 128), (31, 5, 5))]
+         ((100, 128, 128), (31, 5, 5)),
+         # wide planes (half a half-spectrum plane per workgroup)
+         ((24, 256, 256), (9, 21, 7)),
+         ((3, 256, 256), (3, 256, 256)),     # kernel as wide as the plane
+         ((13, 192, 192), (5, 7, 9)),        # two row-pair groups forward
+         ((11, 192, 256), (7, 5, 3)),        # two column groups inverse
+         ((9, 160, 160), (1, 1, 1)),
+         ((17, 128, 256), (4, 6, 8)),
+         ((10, 256, 128), (3, 9, 5)),
+         ((7, 144, 144), (5, 5, 5)),
+         ((12, 240, 240), (11, 3, 7))]
 
 
 def _ids(v):
@@ -90,9 +101,10 @@ def test_plane_path_runs(fc, oracle):
         n = lib.fcamd_pass_times(ms, cnt, 16)
         return {lib.fcamd_pass_name(i).decode(): cnt[i] for i in range(n)}
 
-    plane = passes((16, 128, 128), (5, 5, 5))
-    print("plane path launches", plane, flush=True)
-    assert plane["image_xfwd"] == 1 and plane["image_xinv"] == 1 and plane["zfused"] == 1
-    assert plane["image_yfwd"] == 0 and plane["image_yinv"] == 0
-    other = passes((16, 160, 160), (5, 5, 5))
+    for shape in ((16, 128, 128), (16, 256, 256)):
+        plane = passes(shape, (5, 5, 5))
+        print("plane path launches", shape, plane, flush=True)
+        assert plane["image_xfwd"] == 1 and plane["image_xinv"] == 1 and plane["zfused"] == 1
+        assert plane["image_yfwd"] == 0 and plane["image_yinv"] == 0
+    other = passes((16, 320, 320), (5, 5, 5))
     assert other["image_yfwd"] == 1 and other["image_yinv"] == 1
