@@ -212,11 +212,12 @@ def batched_comparison(fc, torch, checks, reps=5):
     }
 
 
-def adjoint_timing(fc, torch, checks, reps=10):
+def adjoint_timing(fc, torch, checks, reps=50):
     """Differentiation workload, 256x256x128 with a 21^3 PSF: forward
     convolution, image gradient (correlation with the PSF) and PSF gradient
     (correlation of the output gradient with the image on the PSF support),
-    device-resident, HIP events per part."""
+    device-resident, autotuned variants (restored afterwards), HIP events per
+    part."""
     from fourierconvolutioncudalib_amd import synth
     from fourierconvolutioncudalib_amd.autograd import grad_kernel, grad_kernel_workspace_bytes
     shape, kshape = (128, 256, 256), (21, 21, 21)
@@ -242,9 +243,18 @@ def adjoint_timing(fc, torch, checks, reps=10):
                                                stream=stream),
         "grad_kernel_ms": gradk,
     }
-    res = {"workload": "adjoint_256x256x128_k21"}
+    lib = fc.lib()
+    nz, ny, nx = shape
+    keep = {(p, L): lib.fcamd_selected_static_variant(p, L)
+            for p, L in ((0, nx), (1, nx), (2, ny), (3, nz), (4, ny))}
+    if TUNE:
+        fc.autotune(shape, kshape, reps=3, stream=stream)
+    res = {"workload": "adjoint_256x256x128_k21", "variants": fc.selected_variants(shape)}
     for name, fn in parts.items():
         res[name] = timed_calls(torch, stream, fn, reps, warm=3)
+    for (p, L), v in keep.items():
+        if v >= 0:
+            lib.fcamd_select_static_variant(p, L, v)
     res["step_ms"] = res["forward_ms"] + res["grad_image_ms"] + res["grad_kernel_ms"]
     out_h, gi_h, gk_h = out.cpu().numpy(), gi.cpu().numpy(), gk[0].cpu().numpy()
     checks.add("adjoint forward", lambda O: (out_h, O.numpy_fft_conv(img_np, psf_np)))
