@@ -25,7 +25,7 @@ def _replays(torch, g, src, data, outs_fn):
 
 
 @pytest.mark.parametrize("shape,kshape", [((64, 96, 128), (9, 7, 5)), ((40, 50, 60), (5, 3, 7)),
-                                          ((32, 128, 128), (7, 7, 7))])
+                                          ((32, 128, 128), (7, 7, 7)), ((24, 256, 256), (9, 9, 9))])
 def test_plan_and_device_call_in_cuda_graph(fc, oracle, shape, kshape):
     import torch
     from fourierconvolutioncudalib_amd import synth

@@ -9,7 +9,11 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("shape,kshape", [((64, 96, 80), (9, 7, 5)), ((13, 50, 36), (3, 5, 9)),
-                                          ((256, 128, 96), (15, 15, 15)), ((1, 40, 48), (1, 9, 7))])
+                                          ((256, 128, 96), (15, 15, 15)), ((1, 40, 48), (1, 9, 7)),
+                                          # wide planes: both directions split; forward
+                                          # passes with the split inverse
+                                          ((32, 256, 256), (9, 9, 9)),
+                                          ((128, 256, 256), (21, 21, 21))])
 def test_plan_equals_plain_calls(fc, oracle, shape, kshape):
     import torch
     from fourierconvolutioncudalib_amd import synth
