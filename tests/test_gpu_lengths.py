@@ -143,6 +143,30 @@ def test_every_variant(fc, oracle, p):
     print(f"{PASS_NAMES[p]}: {checked} (length, variant, kernel) cases", flush=True)
 
 
+# z extents around the fused z pass's paths for two-stage plans (L = R0 x R1):
+# the tap path up to R1 taps, the kernel column's own transform past that,
+# up to the longest kernels
+Z_PATH_EXTENTS = {64: (8, 9, 17, 32, 33), 128: (8, 9, 21, 64, 65), 240: (15, 16, 61, 64, 65),
+                  256: (16, 17, 40, 64, 65)}
+
+
+@pytest.mark.parametrize("L", sorted(Z_PATH_EXTENTS))
+def test_z_paths_by_kernel_extent(fc, oracle, L):
+    """Fused z pass at kernel extents on both sides of each path boundary,
+    convolution and correlation, odd and even extents (centre kz // 2),
+    against the FP64 oracle."""
+    from fourierconvolutioncudalib_amd import synth
+    for kz in Z_PATH_EXTENTS[L]:
+        print("z path", L, kz, flush=True)
+        img = synth.uniform((L, 6, 18), 40 * L + kz)
+        k = synth.random_psf((kz, 5, 3), 3 * kz + 1)
+        err = oracle.rel_l2(fc.convolve(img, k), oracle.numpy_fft_conv(img, k))
+        assert err < TOL, (L, kz, "convolve", err)
+        out = fc.convolve(img, k, op=fc.CORRELATE)
+        err = oracle.rel_l2(out, oracle.numpy_fft_conv(img, k, conj_kernel=True))
+        assert err < TOL, (L, kz, "correlate", err)
+
+
 @pytest.mark.parametrize("cfg", ["METRIC", "CFG1", "CFG2"])
 def test_tuned_workloads_full_size(fc, oracle, cfg):
     """The plain workloads the benchmark times, at full size, with the
