@@ -1,24 +1,16 @@
 // plane_kernels.hip -- instantiations and dispatch of the whole-plane x/y
-// kernels (plane_kernels.hpp).
+// kernels (plane_kernels.hpp); the wide planes are instantiated in
+// plane_kernels_wide.hip.
 #include <atomic>
 #include <cstdlib>
 
 #include "conv_driver.hpp"
 #include "plane_kernels.hpp"
+#include "plane_registry.hpp"
 #include "static_registry.hpp"
 
 namespace fcamd {
 namespace {
-
-using PlaneFn = void (*)(const PlaneArgs&, hipStream_t);
-using OccFn = int (*)();
-struct PlaneEntry {
-  int nx, ny;
-  PlaneFn fwd, inv;    // one workgroup per plane (null for wide planes)
-  PlaneFn fwd2, inv2;  // split kernels, where the half lengths have plans
-  bool wide;           // only the wide split kernels
-  OccFn occ_fwd2, occ_inv2;  // wide: workgroups per CU of fwd2 / inv2
-};
 
 template <int NX, int NY>
 constexpr PlaneFn split_fwd() {
@@ -38,11 +30,6 @@ constexpr PlaneFn split_inv() {
 #define FC_PLANE(NX, NY)                                                                 \
   {NX, NY, &plane_launch_fwd<NX, NY>, &plane_launch_inv<NX, NY>, split_fwd<NX, NY>(), \
    split_inv<NX, NY>(), false, nullptr, nullptr}
-#define FC_PLANE_WIDE(NX, NY)                                                                      \
-  {NX, NY, nullptr, nullptr, &plane_launch_fwd2w<NX, NY>, &plane_launch_inv2w<NX, NY>, true, \
-   &plane_occupancy_fwd2w<NX, NY>, &plane_occupancy_inv2w<NX, NY>}
-#define FC_PLANE_WIDE_INV(NX, NY) \
-  {NX, NY, nullptr, nullptr, nullptr, &plane_launch_inv2w<NX, NY>, true, nullptr, &plane_occupancy_inv2w<NX, NY>}
 #define FC_PLANE_ROW(NX)                                                          \
   FC_PLANE(NX, 64), FC_PLANE(NX, 80), FC_PLANE(NX, 96), FC_PLANE(NX, 112),        \
       FC_PLANE(NX, 120), FC_PLANE(NX, 128)
@@ -65,21 +52,7 @@ const PlaneEntry kPlanes[] = {
     FC_PLANE(160, 96), FC_PLANE(160, 112), FC_PLANE(192, 64), FC_PLANE(192, 80),
     FC_PLANE(192, 96), FC_PLANE(224, 64), FC_PLANE(224, 80), FC_PLANE(240, 64),
     FC_PLANE(256, 64),
-    // wide planes: half a half-spectrum plane per workgroup (35-150 KB).
-    // 240 x 240 keeps the separate forward passes: its wide forward
-    // measured slower at 96 planes (57.8 -> 61.4 us per convolution),
-    // where the other wide planes gained at 64 planes (256 x 240: 72.1 ->
-    // 64.5 us forward split, 55.8 us both).
-    FC_PLANE_WIDE(128, 160), FC_PLANE_WIDE(160, 128), FC_PLANE_WIDE(144, 144),
-    FC_PLANE_WIDE(160, 160), FC_PLANE_WIDE(128, 192), FC_PLANE_WIDE(192, 128),
-    FC_PLANE_WIDE(160, 192), FC_PLANE_WIDE(192, 160), FC_PLANE_WIDE(192, 192),
-    FC_PLANE_WIDE(224, 224), FC_PLANE_WIDE_INV(240, 240), FC_PLANE_WIDE(256, 256),
-    FC_PLANE_WIDE(128, 256), FC_PLANE_WIDE(256, 128), FC_PLANE_WIDE(192, 256),
-    FC_PLANE_WIDE(256, 192), FC_PLANE_WIDE(224, 256), FC_PLANE_WIDE(256, 224),
-    FC_PLANE_WIDE(240, 256), FC_PLANE_WIDE(256, 240),
 };
-#undef FC_PLANE_WIDE_INV
-#undef FC_PLANE_WIDE
 #undef FC_PLANE_ROW
 #undef FC_PLANE
 
@@ -96,7 +69,7 @@ const PlaneEntry* plane_entry(const ConvShape& s) {
   if (!planes_enabled()) return nullptr;
   for (const PlaneEntry& e : kPlanes)
     if (e.nx == s.nx && e.ny == s.ny) return &e;
-  return nullptr;
+  return wide_plane_entry(s.nx, s.ny);
 }
 
 // FCAMD_PLANE_SPLIT: unset = automatic; 0 = never; 1 = always (where the
