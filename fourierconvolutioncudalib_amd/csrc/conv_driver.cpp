@@ -782,10 +782,21 @@ void enqueue_grad_kernel(const ConvShape& s, float* d_dkernel, const float* d_gr
   z_pass(s, G, I, true, true, st, s.kz, s.cz);
   ConvShape sub = s;
   sub.nz = s.kz;
-  inverse_yx(sub, G, U, st);
-  ConvShape g = sub;
-  g.cz = 0;
-  launch_gather_kernel_support(g, U, d_dkernel, st);
+  const FftDesc fx = fft_desc(s.nx, st);
+  if (fx.M == 0 && gather_support_c2r_lds(s.nx) <= 65536) {
+    // the gather reads kx of each row's nx outputs on ky of its ny rows:
+    // the y inverse of the support planes, then each support row's C2R at
+    // the support positions only (no x inverse pass, no real planes)
+    pass_y_inverse(sub, G, st);
+    launch_gather_support_c2r(s, G, padded_bins(s.nx), fx.tw, d_dkernel, st);
+  } else {
+    // Bluestein x (its table is not the length-nx one the direct C2R reads)
+    // or a row and table past 64 KiB of LDS
+    inverse_yx(sub, G, U, st);
+    ConvShape g = sub;
+    g.cz = 0;
+    launch_gather_kernel_support(g, U, d_dkernel, st);
+  }
   FC_HIP_CHECK(hipGetLastError());
 }
 

@@ -182,6 +182,14 @@ void enqueue_grad_kernel(const ConvShape& s, float* d_dkernel, const float* d_gr
 // dk[jz][jy][jx] = u[(jz-cz) mod nz][(jy-cy) mod ny][(jx-cx) mod nx]
 void launch_gather_kernel_support(const ConvShape& s, const float* u, float* dk,
                                   hipStream_t stream);
+// The same gather from the y-inverted half-spectrum rows of the kz support
+// planes (plane jz holding z-plane (jz - cz) mod nz, row pitch nxp), with the
+// x-C2R of each support row evaluated at the kx support positions only; tw
+// is the length-nx table exp(-2 pi i j / nx).
+void launch_gather_support_c2r(const ConvShape& s, const float2* Y, long nxp, const float2* tw,
+                               float* dk, hipStream_t stream);
+// its LDS bytes (the row, the table and the partial sums)
+size_t gather_support_c2r_lds(int nx);
 
 // Per-device pooled resources for callers that pass no workspace.
 struct DevicePool {

@@ -43,6 +43,67 @@ gather_kernel_support(const float* __restrict__ u, float* __restrict__ dk, int n
   }
 }
 
+// dk[jz][jy][jx] = u[jz][(jy-cy) mod ny][(jx-cx) mod nx] straight from the
+// support planes' y-inverted half-spectrum rows Y (plane jz of Y holds
+// z-plane (jz - cz) mod nz, rotated there by the z pass): one workgroup per
+// support row (jz, jy), each output the row's C2R at one x,
+//   u(x) = Re Y[0] + sum_{0<k<nx/2} 2 Re(Y[k] e^{2 pi i k x / nx})
+//          + Re Y[nx/2] (-1)^x   (even nx),
+// the same Hermitian extension the x-C2R pass transforms.  The row and the
+// table tw[j] = exp(-2 pi i j / nx) are staged in LDS; each output's sum is
+// split over P = blockDim / kx threads and reduced through LDS.
+__global__ void __launch_bounds__(256)
+gather_support_c2r(const float2* __restrict__ Y, long nxp, const float2* __restrict__ tw,
+                   float* __restrict__ dk, int nx, int ny, int kx, int ky, int cx, int cy) {
+  extern __shared__ float2 smem[];
+  const int nb = nx / 2 + 1;
+  float2* row = smem;
+  float2* twl = smem + nb;
+  float* part = reinterpret_cast<float*>(twl + nx);  // blockDim.x partial sums
+  const int jz = blockIdx.x / ky, jy = blockIdx.x - jz * ky;
+  int y = jy - cy;
+  if (y < 0) y += ny;
+  const float2* src = Y + ((long)jz * ny + y) * nxp;
+  for (int k = threadIdx.x; k < nb; k += blockDim.x) row[k] = src[k];
+  for (int k = threadIdx.x; k < nx; k += blockDim.x) twl[k] = tw[k];
+  __syncthreads();
+  const bool even = (nx & 1) == 0;
+  const int kend = even ? nx / 2 : nb;  // paired bins [1, kend)
+  const int P = (int)blockDim.x / kx >= 1 ? (int)blockDim.x / kx : 1;
+  const int per = (int)blockDim.x / P;  // outputs per round
+  const int C = (kend - 1 + P - 1) / P;  // bins per partial sum
+  const int slot = threadIdx.x / P, p = threadIdx.x - slot * P;
+  for (int j0 = 0; j0 < kx; j0 += per) {  // workgroup-uniform rounds
+    const int jx = j0 + slot;
+    const bool act = slot < per && jx < kx;
+    int x = 0;
+    float pair = 0.f;
+    if (act) {
+      x = jx - cx;
+      if (x < 0) x += nx;
+      const int k0 = 1 + p * C, k1 = k0 + C < kend ? k0 + C : kend;
+      int e = (int)(((long)k0 * x) % nx);  // (k x) mod nx
+      for (int k = k0; k < k1; ++k) {
+        // Re(Y e^{+i theta}) with e^{+i theta} = conj(tw[e])
+        const float2 h = row[k], w = twl[e];
+        pair += h.x * w.x + h.y * w.y;
+        e += x;
+        if (e >= nx) e -= nx;
+      }
+    }
+    part[threadIdx.x] = pair;
+    __syncthreads();
+    if (act && p == 0) {
+      float sum = 0.f;
+      for (int q = 0; q < P; ++q) sum += part[threadIdx.x + q];
+      float acc = row[0].x + 2.f * sum;
+      if (even) acc += (x & 1) ? -row[nx / 2].x : row[nx / 2].x;
+      dk[((long)jz * ky + jy) * kx + jx] = acc;
+    }
+    __syncthreads();  // the partials are reused by the next round
+  }
+}
+
 // Richardson-Lucy elementwise steps (unfused iteration and initialisation)
 __global__ void __launch_bounds__(256)
 rl_mean_views(float* __restrict__ psi, const float* const* __restrict__ views, int nviews, long n) {
@@ -106,6 +167,17 @@ void launch_modulate(float2* dst, const float2* src, long n, float scale, int co
   if (blocks < 1) return;
   hipLaunchKernelGGL(modulate_and_normalize, dim3((unsigned)blocks), dim3(256), 0, st, dst, src, n,
                      scale, conj);
+}
+
+size_t gather_support_c2r_lds(int nx) {
+  return sizeof(float2) * ((size_t)(nx / 2 + 1) + (size_t)nx) + sizeof(float) * 256;
+}
+
+void launch_gather_support_c2r(const ConvShape& s, const float2* Y, long nxp, const float2* tw,
+                               float* dk, hipStream_t st) {
+  hipLaunchKernelGGL(gather_support_c2r, dim3((unsigned)(s.kz * s.ky)), dim3(256),
+                     gather_support_c2r_lds(s.nx), st, Y, nxp, tw, dk, s.nx, s.ny, s.kx, s.ky,
+                     s.cx, s.cy);
 }
 
 void launch_gather_kernel_support(const ConvShape& s, const float* u, float* dk, hipStream_t st) {

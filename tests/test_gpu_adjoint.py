@@ -27,6 +27,27 @@ def test_grad_kernel_small_vs_direct(fc, oracle):
         assert oracle.rel_l2(dk.cpu().numpy(), ref) < 1e-5, kshape
 
 
+@pytest.mark.parametrize("shape,kshapes", [
+    ((9, 12, 35), [(3, 5, 7), (9, 12, 35), (4, 6, 2)]),    # odd nx: no Nyquist bin
+    ((6, 10, 74), [(3, 3, 9), (5, 4, 74)]),                # Bluestein nx (37 > 31)
+    ((8, 16, 256), [(5, 7, 21), (3, 16, 256), (1, 1, 1)]),
+])
+def test_grad_kernel_row_c2r_shapes(fc, oracle, shape, kshapes):
+    """PSF gradient against FP64 where the support rows' x-C2R is evaluated
+    directly at the kernel's x positions (odd and even nx, full and
+    one-tap x extents) and where a Bluestein x keeps the x inverse pass."""
+    import torch
+    from fourierconvolutioncudalib_amd.autograd import grad_kernel
+    rng = np.random.default_rng(sum(shape))
+    img = rng.random(shape, dtype=np.float32)
+    g = rng.standard_normal(shape).astype(np.float32)
+    for kshape in kshapes:
+        print("grad_kernel", shape, kshape, flush=True)
+        dk = grad_kernel(torch.from_numpy(g).cuda(), torch.from_numpy(img).cuda(), kshape)
+        ref = oracle.conv_grad_kernel(g.astype(np.float64), img, kshape)
+        assert oracle.rel_l2(dk.cpu().numpy(), ref) < 1e-5, (shape, kshape)
+
+
 def _fft64(a):
     return np.fft.rfftn(np.asarray(a, np.float64))
 
