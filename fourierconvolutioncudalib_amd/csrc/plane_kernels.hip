@@ -72,15 +72,20 @@ const PlaneEntry* plane_entry(const ConvShape& s) {
   return wide_plane_entry(s.nx, s.ny);
 }
 
-// FCAMD_PLANE_SPLIT: unset = automatic; 0 = never; 1 = always (where the
-// half lengths have plans); f / i = the forward / the inverse only
+// FCAMD_PLANE_SPLIT: 0 = never; 1 = always (where the half lengths have
+// plans); f / i = the forward / the inverse only; unset or anything else =
+// automatic
 int split_mode() {
   static const int mode = [] {
     const char* v = std::getenv("FCAMD_PLANE_SPLIT");
-    if (!v || !v[0]) return -1;
-    if (v[0] == 'f') return 1;
-    if (v[0] == 'i') return 2;
-    return v[0] == '0' ? 0 : 3;
+    if (!v || !v[0] || v[1]) return -1;
+    switch (v[0]) {
+      case '0': return 0;
+      case 'f': return 1;
+      case 'i': return 2;
+      case '1': return 3;
+      default: return -1;
+    }
   }();
   return mode;
 }
