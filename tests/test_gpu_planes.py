@@ -8,9 +8,10 @@ take the split kernels (two workgroups per plane: even/odd x bins forward,
 even/odd rows inverse); the cases cover split and whole-plane kernels in
 either direction.  Checked against the FP64 oracle at
 rel-L2 <= 1e-5 (the bar of every convolution here) for every instantiated
-plane extent, convolution and correlation, kernels from one tap to the full
-plane, off-centre and even extents, through the drop-in host entry point
-and the device entry point; the per-pass event timing confirms that no
+plane extent (each in the split or wide regime and, where it has them, on
+the one-workgroup kernels), convolution and correlation, kernels from one
+tap to the full plane, off-centre and even extents, through the drop-in
+host entry point and the device entry point; the per-pass event timing confirms that no
 separate y pass was launched.  Semantics: src/convolution3Dfft.cu:399-578
 (circular convolution, kernel centred at floor(k/2)).
 """
@@ -77,6 +78,55 @@ def test_plane_extents(fc, oracle, shape, kshape):
     got = fc.convolve3d(ti, tk, op=fc.CORRELATE).cpu().numpy()
     ref_c = oracle.numpy_fft_conv(stack, kernel, conj_kernel=True)
     assert oracle.rel_l2(got, ref_c) < TOL
+
+
+# Every plane extent with plane kernels (plane_kernels.hip,
+# plane_kernels_wide.hip): the 69 with one-workgroup kernels and the 20 wide
+# ones, as (nx, ny).
+REGULAR_PLANES = ([(nx, ny) for nx in (64, 80, 96, 112, 120, 128) for ny in (64, 80, 96, 112, 120, 128)]
+                  + [(64, 144), (64, 160), (64, 192), (64, 224), (64, 240), (64, 256), (80, 144),
+                     (80, 160), (80, 192), (80, 224), (96, 144), (96, 160), (112, 144), (112, 160),
+                     (120, 144), (128, 144), (144, 64), (144, 80), (144, 96), (144, 112), (144, 120),
+                     (144, 128), (160, 64), (160, 80), (160, 96), (160, 112), (192, 64), (192, 80),
+                     (192, 96), (224, 64), (224, 80), (240, 64), (256, 64)])
+WIDE_PLANES = [(128, 160), (160, 128), (144, 144), (160, 160), (128, 192), (192, 128), (160, 192),
+               (192, 160), (192, 192), (224, 224), (240, 240), (256, 256), (128, 256), (256, 128),
+               (192, 256), (256, 192), (224, 256), (256, 224), (240, 256), (256, 240)]
+
+
+def _check_planes(fc, oracle, planes, nz, kshape, op, seed):
+    import torch
+    from fourierconvolutioncudalib_amd import synth
+    worst = 0.0
+    for nx, ny in planes:
+        shape = (nz, ny, nx)
+        stack = synth.uniform(shape, seed + nx + 7 * ny)
+        kernel = synth.random_psf(kshape, seed + nx - ny)
+        got = fc.convolve3d(torch.from_numpy(stack).cuda(), torch.from_numpy(kernel).cuda(),
+                            op=op).cpu().numpy()
+        ref = oracle.numpy_fft_conv(stack, kernel, conj_kernel=op == fc.CORRELATE)
+        err = oracle.rel_l2(got, ref)
+        assert err < TOL, (shape, kshape, op, err)
+        worst = max(worst, err)
+    return worst
+
+
+def test_every_plane_extent_few_planes(fc, oracle):
+    """Every registered plane extent at 6 planes and a 3 x 5 x 7 kernel: on
+    256 CUs the split kernels run in both directions where they exist, the
+    wide split kernels for the wide extents, the one-workgroup kernels
+    elsewhere; convolution against the FP64 oracle."""
+    worst = _check_planes(fc, oracle, REGULAR_PLANES + WIDE_PLANES, 6, (3, 5, 7), fc.CONVOLVE, 500)
+    print(f"{len(REGULAR_PLANES) + len(WIDE_PLANES)} extents, worst rel-L2 {worst:.2e}", flush=True)
+
+
+def test_every_regular_plane_extent_many_planes(fc, oracle):
+    """Every extent with one-workgroup kernels at 132 planes and a 1 x 3 x 3
+    kernel: on 256 CUs 2 * (nz + kz) and 2 * nz exceed the CU count, so the
+    one-workgroup kernels run in both directions; correlation against the
+    FP64 oracle."""
+    worst = _check_planes(fc, oracle, REGULAR_PLANES, 132, (1, 3, 3), fc.CORRELATE, 900)
+    print(f"{len(REGULAR_PLANES)} extents, worst rel-L2 {worst:.2e}", flush=True)
 
 
 def test_plane_path_runs(fc, oracle):
