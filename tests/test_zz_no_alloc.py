@@ -28,6 +28,7 @@ def test_noalloc_matches_device_entry_point(fc, oracle):
     cases = [((64, 128, 128), (15, 15, 15), 0),  # whole-plane x/y kernels
              ((32, 256, 256), (9, 9, 9), 0),      # wide planes, half-length tables
              ((64, 256, 256), (21, 21, 21), 1),
+             ((128, 256, 256), (21, 21, 21), 0),  # x and y forward passes, wide inverse
              ((64, 96, 128), (9, 7, 5), 0),   # compile-time lengths
              ((40, 48, 64), (5, 5, 3), 1),    # smaller, correlation
              ((30, 22, 26), (5, 3, 3), 0),    # runtime lengths, smaller again
