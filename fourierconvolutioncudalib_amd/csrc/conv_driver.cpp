@@ -24,6 +24,7 @@
 #include <map>
 #include <memory>
 #include <string>
+#include <tuple>
 
 #include "fft_static.hpp"
 #include "kernels_long.hpp"
@@ -254,7 +255,56 @@ float2* upload_table(const std::vector<float2>& h) {
 
 std::map<std::pair<int, int>, FftDesc> g_desc;  // (device, L) -> descriptor
 
+struct ChirpKey {
+  int dev, L, M, M1;
+  bool operator<(const ChirpKey& o) const {
+    return std::tie(dev, L, M, M1) < std::tie(o.dev, o.L, o.M, o.M1);
+  }
+};
+std::map<ChirpKey, std::pair<const float2*, const float2*>> g_chirp;
+
 }  // namespace
+
+void long_chirp_tables(int L, int M, int M1, hipStream_t st, const float2** chirp,
+                       const float2** bfft) {
+  FC_REQUIRE(M >= 2 * L - 1 && M1 > 0 && M % M1 == 0, kInternal, "bad chirp-z length");
+  const ChirpKey key{current_device(), L, M, M1};
+  {
+    std::lock_guard<std::mutex> lk(g_tw_mu);
+    auto it = g_chirp.find(key);
+    if (it != g_chirp.end()) {
+      *chirp = it->second.first;
+      *bfft = it->second.second;
+      return;
+    }
+  }
+  if (st != nullptr) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    FC_HIP_CHECK(hipStreamIsCapturing(st, &cs));
+    FC_REQUIRE(cs == hipStreamCaptureStatusNone, kInvalidArgument,
+               "transform length " + std::to_string(L) +
+                   " is used for the first time inside a stream capture; run one call of "
+                   "this shape before capturing");
+  }
+  std::vector<float2> c, bf;
+  bluestein_tables(L, M, &c, &bf);
+  // the forward four-step M = M1 * M2 leaves frequency k1 + M1 k2 at
+  // position k1 M2 + k2
+  const int M2 = M / M1;
+  std::vector<float2> perm(M);
+  for (int k1 = 0; k1 < M1; ++k1)
+    for (int k2 = 0; k2 < M2; ++k2) perm[(size_t)k1 * M2 + k2] = bf[(size_t)k1 + (size_t)M1 * k2];
+  float2* dc = upload_table(c);
+  float2* db = upload_table(perm);
+  std::lock_guard<std::mutex> lk(g_tw_mu);
+  auto ins = g_chirp.emplace(key, std::make_pair((const float2*)dc, (const float2*)db));
+  if (!ins.second) {  // another thread made them first: keep one
+    (void)hipFree(dc);
+    (void)hipFree(db);
+  }
+  *chirp = ins.first->second.first;
+  *bfft = ins.first->second.second;
+}
 
 FftDesc fft_desc(int L, hipStream_t st) {
   if (const FftDescSet* o = t_desc_override) {

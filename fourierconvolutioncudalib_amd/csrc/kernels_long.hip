@@ -170,6 +170,51 @@ lk_scatter_kernel(const float* __restrict__ k, float* __restrict__ v, int nx, in
   }
 }
 
+// Bluestein's chirp-z for an axis of length L (no split into two tiles),
+// over lines x(o, p, c) at o * hi + p * elem + c (o < nouter, p < L,
+// c < ncols) and a scratch W[nouter][M][ncols]:
+//   W(o, m, c) = x(o, m, c) * chirp[m] (m < L), 0 (L <= m < M)
+//   W <- FFT_M(W) * bfft (four-step order), then the inverse FFT_M
+//   x(o, k, c) = W(o, k, c) * chirp[k]   (k < L)
+// with conj(chirp) and conj(bfft) for the inverse direction (dir > 0).
+__device__ __forceinline__ float2 chirp_mul(float2 v, float2 w, int dir) {
+  return dir < 0 ? cmul(v, w) : cmulc(v, w);
+}
+
+__global__ void __launch_bounds__(256)
+lk_chirp_pre(const float2* __restrict__ x, long hi, long elem, long ncols, long nouter, int L,
+             float2* __restrict__ W, int M, const float2* __restrict__ chirp, int dir) {
+  const long n = nouter * M * ncols;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x) {
+    const long c = i % ncols, r = i / ncols;
+    const int m = (int)(r % M);
+    const long o = r / M;
+    W[i] = m < L ? chirp_mul(x[o * hi + m * elem + c], chirp[m], dir) : make_float2(0.f, 0.f);
+  }
+}
+
+__global__ void __launch_bounds__(256)
+lk_chirp_mul(float2* __restrict__ W, long ncols, long n, int M, const float2* __restrict__ bfft,
+             int dir) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x)
+    W[i] = chirp_mul(W[i], bfft[(i / ncols) % M], dir);
+}
+
+__global__ void __launch_bounds__(256)
+lk_chirp_post(const float2* __restrict__ W, float2* __restrict__ x, long hi, long elem, long ncols,
+              long nouter, int L, int M, const float2* __restrict__ chirp, int dir) {
+  const long n = nouter * L * ncols;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x) {
+    const long c = i % ncols, r = i / ncols;
+    const int k = (int)(r % L);
+    const long o = r / L;
+    x[o * hi + k * elem + c] = chirp_mul(W[(o * M + k) * ncols + c], chirp[k], dir);
+  }
+}
+
 namespace {
 unsigned ew_grid(long n) {
   long b = (n + 255) / 256;
@@ -207,6 +252,21 @@ void launch_xunpack(const float2* S, float2* z, long nrows, int nx, long srow, i
 void launch_xreal(const float2* z, float* v, long nrows, int nx, hipStream_t st) {
   hipLaunchKernelGGL(lk_xreal, dim3(ew_grid((nrows + 1) / 2 * nx)), dim3(256), 0, st, z, v, nrows,
                      nx);
+}
+void launch_chirp_pre(const float2* x, long hi, long elem, long ncols, long nouter, int L,
+                      float2* W, int M, const float2* chirp, int dir, hipStream_t st) {
+  hipLaunchKernelGGL(lk_chirp_pre, dim3(ew_grid(nouter * M * ncols)), dim3(256), 0, st, x, hi, elem,
+                     ncols, nouter, L, W, M, chirp, dir);
+}
+void launch_chirp_mul(float2* W, long ncols, long nouter, int M, const float2* bfft, int dir,
+                      hipStream_t st) {
+  const long n = nouter * M * ncols;
+  hipLaunchKernelGGL(lk_chirp_mul, dim3(ew_grid(n)), dim3(256), 0, st, W, ncols, n, M, bfft, dir);
+}
+void launch_chirp_post(const float2* W, float2* x, long hi, long elem, long ncols, long nouter,
+                       int L, int M, const float2* chirp, int dir, hipStream_t st) {
+  hipLaunchKernelGGL(lk_chirp_post, dim3(ew_grid(nouter * L * ncols)), dim3(256), 0, st, W, x, hi,
+                     elem, ncols, nouter, L, M, chirp, dir);
 }
 void launch_fill_zero(float* v, long n, hipStream_t st) {
   hipLaunchKernelGGL(lk_fill_zero, dim3(ew_grid(n)), dim3(256), 0, st, v, n);

@@ -37,6 +37,15 @@ void launch_xsep(const float2* z, float2* S, long nrows, int nx, long srow, int 
 void launch_xunpack(const float2* S, float2* z, long nrows, int nx, long srow, int L1, int L2,
                     hipStream_t st);
 void launch_xreal(const float2* z, float* v, long nrows, int nx, hipStream_t st);
+// Bluestein's chirp-z steps for an axis of length L with no split into two
+// tiles (kernels_long.hip): lines x(o, p, c) at o * hi + p * elem + c,
+// scratch W[nouter][M][ncols]; dir -1 forward, +1 inverse
+void launch_chirp_pre(const float2* x, long hi, long elem, long ncols, long nouter, int L,
+                      float2* W, int M, const float2* chirp, int dir, hipStream_t st);
+void launch_chirp_mul(float2* W, long ncols, long nouter, int M, const float2* bfft, int dir,
+                      hipStream_t st);
+void launch_chirp_post(const float2* W, float2* x, long hi, long elem, long ncols, long nouter,
+                       int L, int M, const float2* chirp, int dir, hipStream_t st);
 void launch_fill_zero(float* v, long n, hipStream_t st);
 void launch_scatter_kernel(const float* k, float* v, int nx, int ny, int nz, int kx, int ky,
                            int kz, int cx, int cy, int cz, hipStream_t st);

@@ -11,7 +11,11 @@
 // forward = x-R2C (two real rows per complex row), y-FFT, z-FFT; inverse the
 // reverse.  A four-step axis leaves its frequencies in a transposed order,
 // the same for S and K, which the pointwise product does not see and the
-// inverse undoes.
+// inverse undoes.  A length with no split into two tiles (a prime factor
+// above one tile) runs as Bluestein's chirp-z: a circular convolution of a
+// 7-smooth length M >= 2L - 1 that does split, through a scratch of M
+// positions per line; its result is in natural frequency order.
+#include <algorithm>
 #include <cmath>
 #include <string>
 
@@ -26,16 +30,36 @@ namespace {
 constexpr int kTileLen = 10240;
 
 struct AxisSplit {
-  int L, L1, L2;  // L1 = 0: one tile
+  int L, L1, L2;      // four-step L = L1 * L2; L1 = 0 and M = 0: one tile
+  int M = 0, M1 = 0;  // chirp-z over the four-step M = M1 * (M / M1)
+  bool one_tile() const { return L1 == 0 && M == 0; }
 };
 
-// one tile when L <= tile (unless `force`), else L1 * L2 with both in a tile
+// the divisor nearest sqrt(L) leaving both parts within a tile, or 0
+int two_tile_divisor(long L, int tile) {
+  for (long d = (long)std::sqrt((double)L); d >= 2; --d)
+    if (L % d == 0 && L / d <= tile && d <= tile) return (int)d;
+  return 0;
+}
+
+bool seven_smooth(long m) {
+  for (int f : {2, 3, 5, 7})
+    while (m % f == 0) m /= f;
+  return m == 1;
+}
+
+// one tile when L <= tile (unless `force`), else L1 * L2 with both in a
+// tile, else the chirp-z over the smallest 7-smooth M >= 2L - 1 that splits
 AxisSplit split_axis(int L, int tile, bool force = false) {
   if (L <= tile && !force) return {L, 0, 0};
-  for (int d = (int)std::sqrt((double)L); d >= 2; --d)
-    if (L % d == 0 && L / d <= tile && d <= tile) return {L, d, L / d};
-  throw Error{kUnsupportedSize, "transform length " + std::to_string(L) +
-                                    " exceeds one tile and has no split into two tiles"};
+  if (const int d = two_tile_divisor(L, tile)) return {L, d, L / d};
+  for (long m = 2L * L - 1; m <= (long)tile * tile; ++m)
+    if (seven_smooth(m))
+      if (const int d = two_tile_divisor(m, tile)) return {L, 0, 0, (int)m, d};
+  throw Error{kUnsupportedSize,
+              "transform length " + std::to_string(L) +
+                  " exceeds one tile and has no split into two tiles, nor a chirp-z length "
+                  "that has one"};
 }
 
 size_t align256(size_t b) { return (b + 255) / 256 * 256; }
@@ -64,11 +88,19 @@ LColArgs lcol(int L, long ncols, long nouter, float2* data, long hi, long lo, lo
   return a;
 }
 
+void chirp_z(const AxisSplit& sp, long ncols, long nouter, float2* data, long hi, long elem,
+             int dir, float2* W, hipStream_t st);
+
 // In-place FFT (dir -1 forward, +1 inverse) along an axis of length L with
 // position stride `elem`, over `ncols` contiguous columns per outer row and
-// `nouter` outer rows of stride `hi`.
+// `nouter` outer rows of stride `hi`; W: the chirp-z scratch, ncols * nouter
+// * sp.M positions.
 void axis_fft(const AxisSplit& sp, long ncols, long nouter, float2* data, long hi, long elem,
-              int dir, hipStream_t st) {
+              int dir, float2* W, hipStream_t st) {
+  if (sp.M > 0) {
+    chirp_z(sp, ncols, nouter, data, hi, elem, dir, W, st);
+    return;
+  }
   if (sp.L1 == 0) {
     launch_lcol(lcol(sp.L, ncols, nouter, data, hi, 0, 0, elem, st), dir, st);
     return;
@@ -90,8 +122,12 @@ void axis_fft(const AxisSplit& sp, long ncols, long nouter, float2* data, long h
 }
 
 // Complex FFTs of the packed rows z[npairs][nx] (contiguous rows).
-void row_fft(const AxisSplit& sp, long npairs, float2* z, int dir, hipStream_t st) {
+void row_fft(const AxisSplit& sp, long npairs, float2* z, int dir, float2* W, hipStream_t st) {
   const int nx = sp.L;
+  if (sp.M > 0) {
+    chirp_z(sp, 1, npairs, z, nx, 1, dir, W, st);
+    return;
+  }
   if (sp.L1 == 0) {
     launch_lcol(lcol(nx, 1, npairs, z, nx, 0, 0, 1, st), dir, st);
     return;
@@ -114,9 +150,38 @@ void row_fft(const AxisSplit& sp, long npairs, float2* z, int dir, hipStream_t s
   }
 }
 
+// Bluestein's chirp-z along the axis (kernels_long.hip: launch_chirp_*); the
+// length-M transforms are four-steps, over contiguous rows when ncols = 1
+void chirp_z(const AxisSplit& sp, long ncols, long nouter, float2* data, long hi, long elem,
+             int dir, float2* W, hipStream_t st) {
+  FC_REQUIRE(W != nullptr, kInternal, "chirp-z without its scratch");
+  const float2* chirp = nullptr;
+  const float2* bfft = nullptr;
+  long_chirp_tables(sp.L, sp.M, sp.M1, st, &chirp, &bfft);
+  const AxisSplit m{sp.M, sp.M1, sp.M / sp.M1};
+  launch_chirp_pre(data, hi, elem, ncols, nouter, sp.L, W, sp.M, chirp, dir, st);
+  if (ncols == 1)
+    row_fft(m, nouter, W, -1, nullptr, st);
+  else
+    axis_fft(m, ncols, nouter, W, (long)sp.M * ncols, ncols, -1, nullptr, st);
+  launch_chirp_mul(W, ncols, nouter, sp.M, bfft, dir, st);
+  if (ncols == 1)
+    row_fft(m, nouter, W, +1, nullptr, st);
+  else
+    axis_fft(m, ncols, nouter, W, (long)sp.M * ncols, ncols, +1, nullptr, st);
+  launch_chirp_post(W, data, hi, elem, ncols, nouter, sp.L, sp.M, chirp, dir, st);
+}
+
+AxisSplit x_split(const ConvShape& s) {
+  // the single-tile x pass holds nx + 1 complex values per row pair; past
+  // that the packed rows always take the four-step (whose parts are
+  // column FFTs of at most kTileLen) or the chirp-z
+  return s.nx <= kTileLen - 1 ? AxisSplit{s.nx, 0, 0} : split_axis(s.nx, kTileLen, true);
+}
+
 struct LongLayout {
-  size_t spectrum, kspectrum, kvol, packed;
-  size_t total() const { return spectrum + kspectrum + kvol + packed; }
+  size_t spectrum, kspectrum, kvol, packed, chirpz;
+  size_t total() const { return spectrum + kspectrum + kvol + packed + chirpz; }
 };
 
 LongLayout long_layout(const ConvShape& s) {
@@ -127,42 +192,45 @@ LongLayout long_layout(const ConvShape& s) {
   w.kspectrum = w.spectrum;
   w.kvol = align256(sizeof(float) * (size_t)s.voxels());
   w.packed = s.nx > kTileLen - 1 ? align256(sizeof(float2) * ((rows + 1) / 2) * s.nx) : 0;
+  // one chirp-z scratch for the largest axis that needs one
+  const size_t nb = (size_t)half_bins(s.nx);
+  const AxisSplit sx = x_split(s), sy = split_axis(s.ny, kTileLen), sz = split_axis(s.nz, kTileLen);
+  size_t cz = 0;
+  if (sx.M > 0) cz = std::max(cz, (rows + 1) / 2 * (size_t)sx.M);
+  if (s.ny > 1 && sy.M > 0) cz = std::max(cz, nb * s.nz * (size_t)sy.M);
+  if (s.nz > 1 && sz.M > 0) cz = std::max(cz, nb * s.ny * (size_t)sz.M);
+  w.chirpz = align256(sizeof(float2) * cz);
   return w;
 }
 
-AxisSplit x_split(const ConvShape& s) {
-  // the single-tile x pass holds nx + 1 complex values per row pair; past
-  // that the packed rows always take the four-step (whose parts are
-  // column FFTs of at most kTileLen)
-  return s.nx <= kTileLen - 1 ? AxisSplit{s.nx, 0, 0} : split_axis(s.nx, kTileLen, true);
-}
-
-void forward(const ConvShape& s, const float* v, float2* S, float2* Z, hipStream_t st) {
+// W: the chirp-z scratch (LongLayout::chirpz)
+void forward(const ConvShape& s, const float* v, float2* S, float2* Z, float2* W,
+             hipStream_t st) {
   const long nxp = padded_bins(s.nx), rows = (long)s.ny * s.nz, plane = (long)s.ny * nxp;
   const int nb = half_bins(s.nx);
   const AxisSplit sx = x_split(s);
-  if (sx.L1 == 0) {
+  if (sx.one_tile()) {
     pass_x_forward(s, v, S, nullptr, nullptr, st);
   } else {
     launch_xpack(v, Z, rows, s.nx, st);
-    row_fft(sx, (rows + 1) / 2, Z, -1, st);
+    row_fft(sx, (rows + 1) / 2, Z, -1, W, st);
     launch_xsep(Z, S, rows, s.nx, nxp, sx.L1, sx.L2, st);
   }
-  if (s.ny > 1) axis_fft(split_axis(s.ny, kTileLen), nb, s.nz, S, plane, nxp, -1, st);
-  if (s.nz > 1) axis_fft(split_axis(s.nz, kTileLen), nb, s.ny, S, nxp, plane, -1, st);
+  if (s.ny > 1) axis_fft(split_axis(s.ny, kTileLen), nb, s.nz, S, plane, nxp, -1, W, st);
+  if (s.nz > 1) axis_fft(split_axis(s.nz, kTileLen), nb, s.ny, S, nxp, plane, -1, W, st);
 }
 
-void inverse(const ConvShape& s, float2* S, float* v, float2* Z, hipStream_t st) {
+void inverse(const ConvShape& s, float2* S, float* v, float2* Z, float2* W, hipStream_t st) {
   const long nxp = padded_bins(s.nx), rows = (long)s.ny * s.nz, plane = (long)s.ny * nxp;
   const int nb = half_bins(s.nx);
-  if (s.nz > 1) axis_fft(split_axis(s.nz, kTileLen), nb, s.ny, S, nxp, plane, +1, st);
-  if (s.ny > 1) axis_fft(split_axis(s.ny, kTileLen), nb, s.nz, S, plane, nxp, +1, st);
+  if (s.nz > 1) axis_fft(split_axis(s.nz, kTileLen), nb, s.ny, S, nxp, plane, +1, W, st);
+  if (s.ny > 1) axis_fft(split_axis(s.ny, kTileLen), nb, s.nz, S, plane, nxp, +1, W, st);
   const AxisSplit sx = x_split(s);
-  if (sx.L1 == 0) {
+  if (sx.one_tile()) {
     pass_x_inverse(s, S, v, st);
   } else {
     launch_xunpack(S, Z, rows, s.nx, nxp, sx.L1, sx.L2, st);
-    row_fft(sx, (rows + 1) / 2, Z, +1, st);
+    row_fft(sx, (rows + 1) / 2, Z, +1, W, st);
     launch_xreal(Z, v, rows, s.nx, st);
   }
 }
@@ -199,14 +267,17 @@ void enqueue_convolution_long(const ConvShape& s, float* d_out, const float* d_i
   float2* K = reinterpret_cast<float2*>(base + w.spectrum);
   float* kv = reinterpret_cast<float*>(base + w.spectrum + w.kspectrum);
   float2* Z = reinterpret_cast<float2*>(base + w.spectrum + w.kspectrum + w.kvol);
+  float2* W = w.chirpz ? reinterpret_cast<float2*>(base + w.spectrum + w.kspectrum + w.kvol +
+                                                   w.packed)
+                       : nullptr;
   // the kernel on the image grid, then both spectra
   launch_fill_zero(kv, s.voxels(), st);
   launch_scatter_kernel(d_kernel, kv, s.nx, s.ny, s.nz, s.kx, s.ky, s.kz, s.cx, s.cy, s.cz, st);
-  forward(s, kv, K, Z, st);
-  forward(s, d_image, S, Z, st);
+  forward(s, kv, K, Z, W, st);
+  forward(s, d_image, S, Z, W, st);
   const long n = (long)padded_bins(s.nx) * s.ny * s.nz;
   launch_modulate(S, K, n, (float)(1.0 / (double)s.voxels()), op == Op::kCorrelate ? 1 : 0, st);
-  inverse(s, S, d_out, Z, st);
+  inverse(s, S, d_out, Z, W, st);
   FC_HIP_CHECK(hipGetLastError());
 }
 

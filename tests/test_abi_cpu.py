@@ -69,6 +69,30 @@ def test_workspace_bytes(fc):
         fc.workspace_bytes((8, 8, 8), (9, 3, 3))
 
 
+def test_long_workspace_bytes_chirp_z(fc):
+    """Long-axis workspace (host arithmetic only): spectrum, kernel spectrum,
+    kernel volume, packed x rows (nx > 10239), and the chirp-z scratch of an
+    axis with no split into two tiles (M = 20580 positions per line for the
+    primes 10243, 10247); a prime whose chirp-z length passes two tiles is
+    refused."""
+    def a256(b):
+        return (b + 255) // 256 * 256
+    M = 20580
+    # y prime: nx = 3 -> 2 bins, row pitch 16; lines = bins x nz
+    nz, ny, nx = 2, 10247, 3
+    spec = a256(8 * 16 * ny * nz)
+    n = fc.workspace_bytes((nz, ny, nx), (1, 9, 3))
+    assert n == 2 * spec + a256(4 * nx * ny * nz) + a256(8 * 2 * nz * M)
+    # x prime: 5122 bins, pitch 5136; three packed row pairs of nx, lines = pairs
+    nz, ny, nx = 2, 3, 10243
+    spec = a256(8 * 5136 * ny * nz)
+    n = fc.workspace_bytes((nz, ny, nx), (1, 3, 7))
+    assert n == 2 * spec + a256(4 * nx * ny * nz) + a256(8 * 3 * nx) + a256(8 * 3 * M)
+    with pytest.raises(fc.FCError) as err:
+        fc.workspace_bytes((1, 1, 52428841), (1, 1, 1))
+    assert err.value.code == 2 and "no split" in str(err.value)
+
+
 def test_batch_workspace_lanes(fc, monkeypatch):
     """Batch workspace under each schedule (host arithmetic only): one slot
     per lane, the slot being the lane's largest block; blocks dealt largest

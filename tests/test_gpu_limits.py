@@ -4,8 +4,11 @@ y <= 10240, z <= 6826: one row, column or z column of the transform held in
 FFTs through HBM for an axis beyond one tile, the reference's unfused
 structure otherwise: src/convolution3Dfft.cu:519-547), against the FP64
 oracle.  A length past one tile with no split into two tiles (a prime above
-10240) is rejected with the unsupported-size status before any device work,
-leaving the image untouched; zero extents are rejected as invalid arguments.
+10240) runs as Bluestein's chirp-z over a 7-smooth length that splits
+(20580 = 140 * 147 for the primes here), as cuFFT plans any size.  Only a
+length whose chirp-z length would pass two tiles (2L - 1 > 10240^2) is
+rejected with the unsupported-size status before any device work, leaving
+the image untouched; zero extents are rejected as invalid arguments.
 
 The exact limits 10239 = 3 * 3413 and 6826 = 2 * 3413 run a direct 3413-point
 DFT stage; 10206 = 2 * 3^6 * 7 and 10240 = 2^11 * 5 run butterflies only.
@@ -55,7 +58,11 @@ LONG_CASES = [((2, 3, 10240), (1, 3, 7)),     # x: 80 x 128 four-step
               ((2, 3, 20000), (1, 3, 9)),     # x: 125 x 160
               ((8000, 2, 3), (7, 1, 3)),      # z: 80 x 100
               ((3, 12000, 2), (1, 15, 1)),    # y: 100 x 120
-              ((7000, 1, 10500), (9, 1, 11))]  # x and z past their limits together
+              ((7000, 1, 10500), (9, 1, 11)),  # x and z past their limits together
+              ((2, 3, 10243), (1, 3, 7)),     # x: prime, chirp-z over 140 x 147
+              ((2, 10247, 3), (1, 9, 3)),     # y: prime, chirp-z
+              ((10253, 2, 3), (5, 1, 3)),     # z: prime, chirp-z
+              ((10253, 2, 1), (5, 1, 1))]     # z: chirp-z over single-bin lines (nx = 1)
 
 
 @pytest.mark.parametrize("shape,kshape", LONG_CASES, ids=lambda v: "x".join(map(str, v)))
@@ -77,9 +84,8 @@ def test_past_single_tile_limits(fc, oracle, shape, kshape):
     assert oracle.rel_l2(got, refc) < TOL
 
 
-@pytest.mark.parametrize("shape", [(2, 3, 10243), (2, 10247, 3), (10253, 2, 3)],
-                         ids=lambda v: "x".join(map(str, v)))
-def test_prime_past_one_tile_rejected(fc, shape):
+@pytest.mark.parametrize("shape", [(1, 1, 52428841)], ids=lambda v: "x".join(map(str, v)))
+def test_prime_past_two_tiles_rejected(fc, shape):
     im = np.ones(shape, np.float32)
     k = np.ones((1, 1, 1), np.float32)
     with pytest.raises(fc.FCError) as err:
