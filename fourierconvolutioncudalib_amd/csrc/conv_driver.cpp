@@ -837,7 +837,17 @@ void enqueue_grad_kernel(const ConvShape& s, float* d_dkernel, const float* d_gr
     // the gather reads kx of each row's nx outputs on ky of its ny rows:
     // the y inverse of the support planes, then each support row's C2R at
     // the support positions only (no x inverse pass, no real planes)
-    pass_y_inverse(sub, G, st);
+    // (the y pass over the kz support planes takes its own tile: the tuned
+    // one fits full volumes.  At 256 x 256 x 128 with 21^3 the 16-bin tile
+    // leaves 170 workgroups for 256 CUs, the 8-bin tile twice as many:
+    // PSF gradient 89.1 -> 86.5 us)
+    const FftDesc fy = fft_desc(s.ny, st);
+    if (s.ny > 1) {
+      ensure_kernel_attributes();
+      PassScope ps(kPassYInverse, st);
+      if (!static_ycol_fill(sub, fy, G, +1, cu_count(), st))
+        launch_gk_col(spectrum_column_args(sub, fy, G), +1, sub.nz, st);
+    }
     launch_gather_support_c2r(s, G, padded_bins(s.nx), fx.tw, d_dkernel, st);
   } else {
     // Bluestein x (its table is not the length-nx one the direct C2R reads)

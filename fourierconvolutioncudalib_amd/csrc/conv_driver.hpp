@@ -119,6 +119,9 @@ void z_pass(const ConvShape& s, float2* S, const float2* operand, bool dense, bo
 void inverse_yx(const ConvShape& s, float2* S, float* dst, hipStream_t stream,
                 const VPad* vp = nullptr);
 
+// compute units of the current device (queried once per device)
+int cu_count();
+
 // Whole-plane x and y transforms (plane_kernels.hip) for plane extents whose
 // half-spectrum plane, or half of it, fits one workgroup's LDS: forward_xy's
 // image spectrum and kernel planes (no kernel rows) in one launch,

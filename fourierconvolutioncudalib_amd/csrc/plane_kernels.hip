@@ -90,7 +90,8 @@ int split_mode() {
   return mode;
 }
 
-// compute units of the current device (queried once per device)
+}  // namespace
+
 int cu_count() {
   constexpr int kDevices = 64;
   static std::atomic<int> cache[kDevices];
@@ -104,6 +105,8 @@ int cu_count() {
   }
   return n;
 }
+
+namespace {
 
 // The split halves each workgroup's work but doubles the workgroups: it pays
 // while both parts of every plane run in one wave (measured on MI355X, 256

@@ -181,6 +181,37 @@ bool static_ycol(const ConvShape& s, const FftDesc& fy, float2* S, int dir, cons
   return true;
 }
 
+bool static_ycol_fill(const ConvShape& s, const FftDesc& fy, float2* S, int dir, int cus,
+                      hipStream_t st) {
+  StaticEntry* e = static_lookup(s.ny);
+  if (!e) return false;
+  const long nb = half_bins(s.nx);
+  int best = -1, smallest = -1;
+  for (int i = 0; i < (int)e->col.size(); ++i) {
+    const int t = e->col[i].tile;
+    if (t <= 0) continue;
+    if (smallest < 0 || t < e->col[smallest].tile) smallest = i;
+    if ((nb + t - 1) / t * s.nz >= cus && (best < 0 || t > e->col[best].tile)) best = i;
+  }
+  if (best < 0) best = smallest;
+  if (best < 0) return static_ycol(s, fy, S, dir, nullptr, nullptr, st);
+  const long nxp = padded_bins(s.nx);
+  Col2 a{};
+  a.m.fft = fy;
+  a.m.nbins = nb;
+  a.m.src = S;
+  a.m.dst = S;
+  a.m.src_outer = a.m.dst_outer = (long)s.ny * nxp;
+  a.m.src_elem = a.m.dst_elem = nxp;
+  a.m.nvalid = s.ny;
+  a.m.center = 0;
+  a.m_outer = s.nz;
+  a.k = a.m;
+  a.k_outer = 0;
+  if (!e->col[best].fn(a, dir, st)) e->col[0].fn(a, dir, st);
+  return true;
+}
+
 bool static_zfused(const ConvShape& s, const ZArgs& a, hipStream_t st) {
   StaticEntry* e = static_lookup(s.nz);
   if (!e) return false;

@@ -20,6 +20,12 @@ bool static_xinv(const ConvShape& s, const FftDesc& fx, const float2* src, float
 // turns the kernel rows into the kernel planes `kplanes`.
 bool static_ycol(const ConvShape& s, const FftDesc& fy, float2* S, int dir, const float2* krows,
                  float2* kplanes, hipStream_t st);
+// The same y-FFT of the spectrum (no kernel rows) on the plain column
+// variant with the largest tile that still gives each of `cus` compute units
+// a workgroup, or the smallest tile when none does, whatever the tuner
+// selected: for passes over a few planes, which the tuner does not time.
+bool static_ycol_fill(const ConvShape& s, const FftDesc& fy, float2* S, int dir, int cus,
+                      hipStream_t st);
 bool static_zfused(const ConvShape& s, const ZArgs& a, hipStream_t st);
 void static_kernels_allow_large_lds();
 

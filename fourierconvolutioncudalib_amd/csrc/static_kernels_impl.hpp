@@ -1211,13 +1211,13 @@ StaticEntry make_static_entry() {
             {"w256-ldsmirror", &sk_launch_xinv_lds<L, 256>},
             {"w256-flat", &sk_launch_xinv_flat<L, 256>},
             {"w128-flat", &sk_launch_xinv_flat<L, 128>}};
-  e.col = {{"t16", &sk_launch_col<L, 16>},
+  e.col = {{"t16", &sk_launch_col<L, 16>, fit_tile<L, 16>()},
            {"t16-pipe", &sk_launch_col_pipe<L, 16>},
            {"t8-pipe", &sk_launch_col_pipe<L, 8>},
            {"t16-revradix", &sk_launch_col<L, 16, true>},
-           {"t8", &sk_launch_col<L, 8>},
-           {"t32", &sk_launch_col<L, 32>},
-           {"t64", &sk_launch_col<L, 64>}};
+           {"t8", &sk_launch_col<L, 8>, fit_tile<L, 8>()},
+           {"t32", &sk_launch_col<L, 32>, fit_tile<L, 32>()},
+           {"t64", &sk_launch_col<L, 64>, fit_tile<L, 64>()}};
   e.zfused = {{"t16-fft", &sk_launch_zfused_fft<L, 16>},
               {"t16-dft", &sk_launch_zfused_dft<L, 16>},
               {"t8-fft", &sk_launch_zfused_fft<L, 8>},

@@ -25,6 +25,7 @@ template <class F>
 struct Variant {
   const char* name;
   F fn;
+  int tile = 0;  // column variants: bins per workgroup of a plain tile (0: other forms)
 };
 
 // batched forms: one launch over a task table {argument record, tile}
