@@ -265,9 +265,17 @@ std::map<ChirpKey, std::pair<const float2*, const float2*>> g_chirp;
 
 }  // namespace
 
+int direct_stage_prime(int L) {
+  if (L < 2) return 0;
+  const int p = largest_prime_factor(L);
+  if (p <= kBluesteinPrime || smooth_at_least(2 * L - 1) <= kBluesteinMaxM) return 0;
+  return p;
+}
+
 void long_chirp_tables(int L, int M, int M1, hipStream_t st, const float2** chirp,
                        const float2** bfft) {
-  FC_REQUIRE(M >= 2 * L - 1 && M1 > 0 && M % M1 == 0, kInternal, "bad chirp-z length");
+  FC_REQUIRE(M >= 2 * L - 1 && M1 >= 0 && (M1 == 0 || M % M1 == 0), kInternal,
+             "bad chirp-z length");
   const ChirpKey key{current_device(), L, M, M1};
   {
     std::lock_guard<std::mutex> lk(g_tw_mu);
@@ -289,11 +297,14 @@ void long_chirp_tables(int L, int M, int M1, hipStream_t st, const float2** chir
   std::vector<float2> c, bf;
   bluestein_tables(L, M, &c, &bf);
   // the forward four-step M = M1 * M2 leaves frequency k1 + M1 k2 at
-  // position k1 M2 + k2
-  const int M2 = M / M1;
-  std::vector<float2> perm(M);
-  for (int k1 = 0; k1 < M1; ++k1)
-    for (int k2 = 0; k2 < M2; ++k2) perm[(size_t)k1 * M2 + k2] = bf[(size_t)k1 + (size_t)M1 * k2];
+  // position k1 M2 + k2 (M1 = 0: one tile, natural order)
+  std::vector<float2> perm(bf);
+  if (M1 > 0) {
+    const int M2 = M / M1;
+    for (int k1 = 0; k1 < M1; ++k1)
+      for (int k2 = 0; k2 < M2; ++k2)
+        perm[(size_t)k1 * M2 + k2] = bf[(size_t)k1 + (size_t)M1 * k2];
+  }
   float2* dc = upload_table(c);
   float2* db = upload_table(perm);
   std::lock_guard<std::mutex> lk(g_tw_mu);

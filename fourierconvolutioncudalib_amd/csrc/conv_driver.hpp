@@ -168,12 +168,17 @@ void enqueue_convolution_long(const ConvShape& s, float* d_out, const float* d_i
 // exp(-2 pi i j / L), j < L, on the current device (created once; a first
 // use while `stream` captures is refused like fft_desc's)
 const float2* long_twiddle_table(int L, hipStream_t stream);
+// The largest prime factor of L when the one-tile plan of L runs it as a
+// direct stage (above 31, with Bluestein's length M past 6826), else 0.
+int direct_stage_prime(int L);
 // Tables of a length-L transform run as Bluestein's chirp-z over a circular
 // convolution of length M = M1 * M2 (M >= 2L - 1, transformed by the
-// four-step of that split): chirp[n] = exp(-i pi n^2 / L) for n < L, and
-// bfft the length-M spectrum of the conjugate chirp sequence, scaled by 1/M
-// and stored in the forward four-step's output order (frequency k1 + M1 k2
-// at position k1 M2 + k2).  Created once per device like the twiddle tables.
+// four-step of that split, or in one tile when M1 = 0): chirp[n] =
+// exp(-i pi n^2 / L) for n < L, and bfft the length-M spectrum of the
+// conjugate chirp sequence, scaled by 1/M and stored in the forward
+// four-step's output order (frequency k1 + M1 k2 at position k1 M2 + k2;
+// natural order when M1 = 0).  Created once per device like the twiddle
+// tables.
 void long_chirp_tables(int L, int M, int M1, hipStream_t stream, const float2** chirp,
                        const float2** bfft);
 // dst <- scale * dst * src (conj(src) when conj != 0) over n complex values

@@ -12,9 +12,11 @@
 // reverse.  A four-step axis leaves its frequencies in a transposed order,
 // the same for S and K, which the pointwise product does not see and the
 // inverse undoes.  A length with no split into two tiles (a prime factor
-// above one tile) runs as Bluestein's chirp-z: a circular convolution of a
-// 7-smooth length M >= 2L - 1 that does split, through a scratch of M
-// positions per line; its result is in natural frequency order.
+// above one tile), or one whose one-tile plan would run a direct stage of a
+// prime above kChirpPrime, runs as Bluestein's chirp-z: a circular
+// convolution of a 7-smooth length M >= 2L - 1 (in one tile, or split into
+// two), through a scratch of M positions per line; its result is in natural
+// frequency order.
 #include <algorithm>
 #include <cmath>
 #include <string>
@@ -28,6 +30,17 @@ namespace {
 // longest complex length the runtime-length engine holds in one tile (two
 // LDS buffers of 160 KiB)
 constexpr int kTileLen = 10240;
+
+// A direct p-point stage costs p complex multiply-adds per output.  On
+// MI355X, x axes of L x 64 x 8 volumes (5^3 kernel) on their direct stages
+// took 32.7 ms at 4099 and 206 ms at 9973 (0.47 and 1.15 ms on the
+// chirp-z), 0.67-1.14 ms at 3431 = 47 * 73, 3599 = 59 * 61, 3737 = 37 * 101
+// and 4661 = 59 * 79 (about 0.45 ms on the chirp-z at these lengths), and
+// 0.30 ms at 3700 = 37 * 100.  Axes whose one-tile plan would run a direct
+// stage of a prime above this take the chirp-z.
+constexpr int kChirpPrime = 41;
+
+bool chirp_length(int L) { return direct_stage_prime(L) > kChirpPrime; }
 
 struct AxisSplit {
   int L, L1, L2;      // four-step L = L1 * L2; L1 = 0 and M = 0: one tile
@@ -49,13 +62,19 @@ bool seven_smooth(long m) {
 }
 
 // one tile when L <= tile (unless `force`), else L1 * L2 with both in a
-// tile, else the chirp-z over the smallest 7-smooth M >= 2L - 1 that splits
+// tile, else the chirp-z over the smallest 7-smooth M >= 2L - 1 in one tile
+// or split into two; lengths with a large direct prime stage go to the
+// chirp-z straight away
 AxisSplit split_axis(int L, int tile, bool force = false) {
-  if (L <= tile && !force) return {L, 0, 0};
-  if (const int d = two_tile_divisor(L, tile)) return {L, d, L / d};
+  if (!chirp_length(L)) {
+    if (L <= tile && !force) return {L, 0, 0};
+    if (const int d = two_tile_divisor(L, tile)) return {L, d, L / d};
+  }
   for (long m = 2L * L - 1; m <= (long)tile * tile; ++m)
-    if (seven_smooth(m))
+    if (seven_smooth(m)) {
+      if (m <= tile) return {L, 0, 0, (int)m, 0};
       if (const int d = two_tile_divisor(m, tile)) return {L, 0, 0, (int)m, d};
+    }
   throw Error{kUnsupportedSize,
               "transform length " + std::to_string(L) +
                   " exceeds one tile and has no split into two tiles, nor a chirp-z length "
@@ -158,7 +177,7 @@ void chirp_z(const AxisSplit& sp, long ncols, long nouter, float2* data, long hi
   const float2* chirp = nullptr;
   const float2* bfft = nullptr;
   long_chirp_tables(sp.L, sp.M, sp.M1, st, &chirp, &bfft);
-  const AxisSplit m{sp.M, sp.M1, sp.M / sp.M1};
+  const AxisSplit m{sp.M, sp.M1, sp.M1 ? sp.M / sp.M1 : 0};
   launch_chirp_pre(data, hi, elem, ncols, nouter, sp.L, W, sp.M, chirp, dir, st);
   if (ncols == 1)
     row_fft(m, nouter, W, -1, nullptr, st);
@@ -176,7 +195,8 @@ AxisSplit x_split(const ConvShape& s) {
   // the single-tile x pass holds nx + 1 complex values per row pair; past
   // that the packed rows always take the four-step (whose parts are
   // column FFTs of at most kTileLen) or the chirp-z
-  return s.nx <= kTileLen - 1 ? AxisSplit{s.nx, 0, 0} : split_axis(s.nx, kTileLen, true);
+  return s.nx <= kTileLen - 1 && !chirp_length(s.nx) ? AxisSplit{s.nx, 0, 0}
+                                                      : split_axis(s.nx, kTileLen, true);
 }
 
 struct LongLayout {
@@ -191,10 +211,10 @@ LongLayout long_layout(const ConvShape& s) {
   w.spectrum = align256(sizeof(float2) * nxp * rows);
   w.kspectrum = w.spectrum;
   w.kvol = align256(sizeof(float) * (size_t)s.voxels());
-  w.packed = s.nx > kTileLen - 1 ? align256(sizeof(float2) * ((rows + 1) / 2) * s.nx) : 0;
+  const AxisSplit sx = x_split(s), sy = split_axis(s.ny, kTileLen), sz = split_axis(s.nz, kTileLen);
+  w.packed = sx.one_tile() ? 0 : align256(sizeof(float2) * ((rows + 1) / 2) * s.nx);
   // one chirp-z scratch for the largest axis that needs one
   const size_t nb = (size_t)half_bins(s.nx);
-  const AxisSplit sx = x_split(s), sy = split_axis(s.ny, kTileLen), sz = split_axis(s.nz, kTileLen);
   size_t cz = 0;
   if (sx.M > 0) cz = std::max(cz, (rows + 1) / 2 * (size_t)sx.M);
   if (s.ny > 1 && sy.M > 0) cz = std::max(cz, nb * s.nz * (size_t)sy.M);
@@ -235,10 +255,38 @@ void inverse(const ConvShape& s, float2* S, float* v, float2* Z, float2* W, hipS
   }
 }
 
+// Creates every table the path reads before its first launch, so that a
+// first use inside a stream capture is refused with nothing recorded.
+void prepare_tables(const ConvShape& s, hipStream_t st) {
+  auto split_tables = [&](int L, int L1) {
+    if (L1 > 0) {
+      (void)fft_desc(L1, st);
+      (void)fft_desc(L / L1, st);
+      (void)long_twiddle_table(L, st);
+    } else {
+      (void)fft_desc(L, st);
+    }
+  };
+  auto axis_tables = [&](const AxisSplit& a) {
+    if (a.M > 0) {
+      const float2* c = nullptr;
+      const float2* b = nullptr;
+      long_chirp_tables(a.L, a.M, a.M1, st, &c, &b);
+      split_tables(a.M, a.M1);
+    } else {
+      split_tables(a.L, a.L1);
+    }
+  };
+  axis_tables(x_split(s));
+  if (s.ny > 1) axis_tables(split_axis(s.ny, kTileLen));
+  if (s.nz > 1) axis_tables(split_axis(s.nz, kTileLen));
+}
+
 }  // namespace
 
 bool needs_long_path(const ConvShape& s) {
-  return s.nx > kTileLen - 1 || s.ny > kTileLen || s.nz > 6826 || s.ny > 65535 || s.nz > 65535;
+  return s.nx > kTileLen - 1 || s.ny > kTileLen || s.nz > 6826 || s.ny > 65535 ||
+         s.nz > 65535 || chirp_length(s.nx) || chirp_length(s.ny) || chirp_length(s.nz);
 }
 
 void validate_long_shape(const ConvShape& s) {
@@ -261,6 +309,7 @@ void enqueue_convolution_long(const ConvShape& s, float* d_out, const float* d_i
                               const float* d_kernel, Op op, void* ws, hipStream_t st) {
   validate_long_shape(s);
   driver_init();
+  prepare_tables(s, st);
   const LongLayout w = long_layout(s);
   char* base = static_cast<char*>(ws);
   float2* S = reinterpret_cast<float2*>(base);

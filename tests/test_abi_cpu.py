@@ -88,6 +88,15 @@ def test_long_workspace_bytes_chirp_z(fc):
     spec = a256(8 * 5136 * ny * nz)
     n = fc.workspace_bytes((nz, ny, nx), (1, 3, 7))
     assert n == 2 * spec + a256(4 * nx * ny * nz) + a256(8 * 3 * nx) + a256(8 * 3 * M)
+    # x = 4099 fits one tile, but its plan would hold a direct 4099-point
+    # stage: the long-axis layout with packed rows and a one-tile chirp-z
+    # (M = 8232); 3700 = 37 * 100 keeps its direct stage and the fused layout
+    nz, ny, nx = 8, 64, 4099
+    spec = a256(8 * 2064 * ny * nz)
+    n = fc.workspace_bytes((nz, ny, nx), (3, 5, 7))
+    assert n == (2 * spec + a256(4 * nx * ny * nz) + a256(8 * 256 * nx) + a256(8 * 256 * 8232))
+    n = fc.workspace_bytes((8, 64, 3700), (3, 5, 7))
+    assert n == a256(8 * 1856 * 64 * 8) + a256(8 * 1856 * 3 * 5) + a256(8 * 1856 * 3 * 64)
     with pytest.raises(fc.FCError) as err:
         fc.workspace_bytes((1, 1, 52428841), (1, 1, 1))
     assert err.value.code == 2 and "no split" in str(err.value)

@@ -53,8 +53,9 @@ def test_runtime_length_kernels(fc, oracle, shape):
 
 # lengths whose largest prime factor exceeds 31 run Bluestein's chirp-z
 # (a circular convolution of 7-smooth length M >= 2L - 1, M <= 6826);
-# 4091 (M > 6826) keeps the direct prime stage
-@pytest.mark.parametrize("L", [37, 79, 109, 173, 346, 1009, 2018, 4091])
+# past that, 3700 = 37 * 100 keeps its direct 37-point stage (primes up to
+# 41) and 3737 = 37 * 101 and 4091 take the chirp-z of the long-axis path
+@pytest.mark.parametrize("L", [37, 79, 109, 173, 346, 1009, 2018, 3700, 3737, 4091])
 @pytest.mark.parametrize("axis", ["x", "y", "z"])
 def test_bluestein_lengths(fc, oracle, axis, L):
     from fourierconvolutioncudalib_amd import synth

@@ -10,8 +10,11 @@ length whose chirp-z length would pass two tiles (2L - 1 > 10240^2) is
 rejected with the unsupported-size status before any device work, leaving
 the image untouched; zero extents are rejected as invalid arguments.
 
-The exact limits 10239 = 3 * 3413 and 6826 = 2 * 3413 run a direct 3413-point
-DFT stage; 10206 = 2 * 3^6 * 7 and 10240 = 2^11 * 5 run butterflies only.
+The exact limits 10239 = 3 * 3413 and 6826 = 2 * 3413 would run a direct
+3413-point DFT stage in one tile; a direct stage of a prime above 41 is
+replaced by the long-axis path's chirp-z (an axis of 4099 points took 32.7 ms
+on the direct stage against 0.12 ms at 4096), so they run there.  10206 =
+2 * 3^6 * 7 and 10240 = 2^11 * 5 run butterflies only.
 Tolerance: rel-L2 <= 1e-5 against the FP64 oracle, as for every other shape.
 """
 import numpy as np
@@ -33,10 +36,10 @@ def _run(fc, stack, kernel):
 
 
 # (image C-order shape, kernel C-order shape)
-CASES = [((2, 3, 10239), (1, 3, 7)),    # x at its limit (prime stage 3413)
+CASES = [((2, 3, 10239), (1, 3, 7)),    # x at its limit (prime factor 3413: chirp-z)
          ((2, 3, 10206), (1, 3, 9)),    # x near its limit, butterflies only
          ((2, 10240, 3), (1, 9, 3)),    # y at its limit
-         ((6826, 2, 3), (5, 1, 3)),     # z at its limit (prime stage 3413)
+         ((6826, 2, 3), (5, 1, 3)),     # z at its limit (prime factor 3413: chirp-z)
          ((6720, 2, 2), (7, 1, 1))]     # z near its limit, butterflies only
 
 
@@ -82,6 +85,32 @@ def test_past_single_tile_limits(fc, oracle, shape, kshape):
     got = fc.convolve3d(ti, tk, op=fc.CORRELATE, workspace=ws).cpu().numpy()
     refc = oracle.numpy_fft_conv(stack, kernel, conj_kernel=True)
     assert oracle.rel_l2(got, refc) < TOL
+
+
+# one-tile lengths whose plan would hold a direct stage of a prime above 41:
+# the chirp-z of the long-axis path (M = 8232 in one tile for 4099; M = 20000
+# = 125 x 160 for 9973 and 16464 for 8198, over a four-step)
+PRIME_FACTOR_CASES = [((8, 64, 4099), (3, 5, 7)),   # x
+                      ((8, 4099, 6), (3, 5, 5)),    # y
+                      ((4099, 8, 6), (7, 5, 5)),    # z
+                      ((3, 5, 9973), (1, 3, 5)),    # x, chirp-z over a four-step
+                      ((6, 3, 8198), (3, 3, 5))]    # x = 2 * 4099
+
+
+@pytest.mark.parametrize("shape,kshape", PRIME_FACTOR_CASES, ids=lambda v: "x".join(map(str, v)))
+def test_large_prime_factor_takes_chirp_z(fc, oracle, shape, kshape):
+    import torch
+    from fourierconvolutioncudalib_amd import synth
+    print("prime factor case", shape, kshape, flush=True)
+    stack = synth.uniform(shape, 4500 + sum(shape))
+    kernel = synth.random_psf(kshape, 4600 + sum(kshape))
+    out = _run(fc, stack, kernel)
+    assert oracle.rel_l2(out, oracle.numpy_fft_conv(stack, kernel)) < TOL
+    ti = torch.from_numpy(stack).cuda()
+    tk = torch.from_numpy(kernel).cuda()
+    ws = torch.empty(fc.workspace_bytes(shape, kshape), dtype=torch.uint8, device="cuda")
+    got = fc.convolve3d(ti, tk, op=fc.CORRELATE, workspace=ws).cpu().numpy()
+    assert oracle.rel_l2(got, oracle.numpy_fft_conv(stack, kernel, conj_kernel=True)) < TOL
 
 
 @pytest.mark.parametrize("shape", [(1, 1, 52428841)], ids=lambda v: "x".join(map(str, v)))
