@@ -165,6 +165,13 @@ void validate_long_shape(const ConvShape& s);
 size_t long_workspace_bytes(const ConvShape& s);
 void enqueue_convolution_long(const ConvShape& s, float* d_out, const float* d_image,
                               const float* d_kernel, Op op, void* ws, hipStream_t stream);
+// its two halves, for plans: the kernel's spectrum into the workspace (every
+// table the path reads is created first), then the image's forward
+// transform, the product with that spectrum and the inverse
+void long_kernel_spectrum(const ConvShape& s, const float* d_kernel, void* ws,
+                          hipStream_t stream);
+void long_apply_spectrum(const ConvShape& s, float* d_out, const float* d_image, Op op, void* ws,
+                         hipStream_t stream);
 // exp(-2 pi i j / L), j < L, on the current device (created once; a first
 // use while `stream` captures is refused like fft_desc's)
 const float2* long_twiddle_table(int L, hipStream_t stream);

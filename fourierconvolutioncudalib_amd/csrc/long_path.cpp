@@ -305,29 +305,53 @@ void validate_long_shape(const ConvShape& s) {
 
 size_t long_workspace_bytes(const ConvShape& s) { return long_layout(s).total(); }
 
-void enqueue_convolution_long(const ConvShape& s, float* d_out, const float* d_image,
-                              const float* d_kernel, Op op, void* ws, hipStream_t st) {
+namespace {
+struct LongRegions {
+  float2 *S, *K;
+  float* kv;
+  float2 *Z, *W;
+};
+LongRegions long_regions(const ConvShape& s, void* ws) {
+  const LongLayout w = long_layout(s);
+  char* base = static_cast<char*>(ws);
+  LongRegions r{};
+  r.S = reinterpret_cast<float2*>(base);
+  r.K = reinterpret_cast<float2*>(base + w.spectrum);
+  r.kv = reinterpret_cast<float*>(base + w.spectrum + w.kspectrum);
+  r.Z = reinterpret_cast<float2*>(base + w.spectrum + w.kspectrum + w.kvol);
+  r.W = w.chirpz ? reinterpret_cast<float2*>(base + w.spectrum + w.kspectrum + w.kvol + w.packed)
+                 : nullptr;
+  return r;
+}
+}  // namespace
+
+void long_kernel_spectrum(const ConvShape& s, const float* d_kernel, void* ws, hipStream_t st) {
   validate_long_shape(s);
   driver_init();
   prepare_tables(s, st);
-  const LongLayout w = long_layout(s);
-  char* base = static_cast<char*>(ws);
-  float2* S = reinterpret_cast<float2*>(base);
-  float2* K = reinterpret_cast<float2*>(base + w.spectrum);
-  float* kv = reinterpret_cast<float*>(base + w.spectrum + w.kspectrum);
-  float2* Z = reinterpret_cast<float2*>(base + w.spectrum + w.kspectrum + w.kvol);
-  float2* W = w.chirpz ? reinterpret_cast<float2*>(base + w.spectrum + w.kspectrum + w.kvol +
-                                                   w.packed)
-                       : nullptr;
-  // the kernel on the image grid, then both spectra
-  launch_fill_zero(kv, s.voxels(), st);
-  launch_scatter_kernel(d_kernel, kv, s.nx, s.ny, s.nz, s.kx, s.ky, s.kz, s.cx, s.cy, s.cz, st);
-  forward(s, kv, K, Z, W, st);
-  forward(s, d_image, S, Z, W, st);
-  const long n = (long)padded_bins(s.nx) * s.ny * s.nz;
-  launch_modulate(S, K, n, (float)(1.0 / (double)s.voxels()), op == Op::kCorrelate ? 1 : 0, st);
-  inverse(s, S, d_out, Z, W, st);
+  const LongRegions r = long_regions(s, ws);
+  // the kernel on the image grid, then its spectrum
+  launch_fill_zero(r.kv, s.voxels(), st);
+  launch_scatter_kernel(d_kernel, r.kv, s.nx, s.ny, s.nz, s.kx, s.ky, s.kz, s.cx, s.cy, s.cz, st);
+  forward(s, r.kv, r.K, r.Z, r.W, st);
   FC_HIP_CHECK(hipGetLastError());
+}
+
+void long_apply_spectrum(const ConvShape& s, float* d_out, const float* d_image, Op op, void* ws,
+                         hipStream_t st) {
+  const LongRegions r = long_regions(s, ws);
+  forward(s, d_image, r.S, r.Z, r.W, st);
+  const long n = (long)padded_bins(s.nx) * s.ny * s.nz;
+  launch_modulate(r.S, r.K, n, (float)(1.0 / (double)s.voxels()), op == Op::kCorrelate ? 1 : 0,
+                  st);
+  inverse(s, r.S, d_out, r.Z, r.W, st);
+  FC_HIP_CHECK(hipGetLastError());
+}
+
+void enqueue_convolution_long(const ConvShape& s, float* d_out, const float* d_image,
+                              const float* d_kernel, Op op, void* ws, hipStream_t st) {
+  long_kernel_spectrum(s, d_kernel, ws, st);
+  long_apply_spectrum(s, d_out, d_image, op, ws, st);
 }
 
 }  // namespace fcamd

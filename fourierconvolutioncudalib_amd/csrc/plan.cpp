@@ -7,6 +7,8 @@
 // TODO ("reuse the results", src/convolution3Dfft.cu:265,363).  The kernel's
 // z spectrum is not stored: the fused z pass derives it per column tile,
 // which moves fewer HBM bytes than reading an Nc-sized spectrum would.
+// Shapes the drop-in call sends to the long-axis path (axes past one tile,
+// large prime factors) keep the kernel's whole spectrum instead.
 #include "../../include/fourierconvolution_amd.h"
 #include "conv_driver.hpp"
 
@@ -19,11 +21,16 @@ struct ConvPlan {
   Op op = Op::kConvolve;
   void* ws = nullptr;
   bool own = false;
+  bool lng = false;  // long-axis path: the kernel's spectrum in the long layout
   float2 *S = nullptr, *Kx = nullptr, *Kxy = nullptr;
 };
 
 void plan_kernel(ConvPlan& p, const float* d_kernel, hipStream_t st) {
   FC_REQUIRE(d_kernel != nullptr, kInvalidArgument, "null kernel pointer");
+  if (p.lng) {
+    long_kernel_spectrum(p.s, d_kernel, p.ws, st);
+    return;
+  }
   // the kernel's rows and planes only (no image in these launches), by the
   // same kernels a plain call of this shape uses, so that executing the
   // plan reproduces the plain call bit for bit
@@ -66,29 +73,37 @@ void* fcamd_conv_plan_create(const int* imDim, const int* kernelDim, const float
     FC_REQUIRE(op == FCAMD_CONVOLVE || op == FCAMD_CORRELATE, kInvalidArgument, "unknown op");
     ConvShape s{imDim[0], imDim[1], imDim[2], kernelDim[0], kernelDim[1], kernelDim[2],
                 kernelDim[0] / 2, kernelDim[1] / 2, kernelDim[2] / 2};
-    validate_shape(s);
+    const bool lng = needs_long_path(s);
+    if (lng)
+      validate_long_shape(s);
+    else
+      validate_shape(s);
     driver_init();
     auto* p = new ConvPlan();
     p->s = s;
+    p->lng = lng;
     p->op = op == FCAMD_CORRELATE ? Op::kCorrelate : Op::kConvolve;
     try {
       FC_HIP_CHECK(hipGetDevice(&p->dev));
-      const WorkspaceLayout wl = workspace_layout(s);
+      const size_t need = lng ? long_workspace_bytes(s) : workspace_layout(s).total();
       if (d_workspace) {
-        FC_REQUIRE(workspace_bytes >= (long long)wl.total(), kInvalidArgument,
-                   "workspace too small");
+        FC_REQUIRE(workspace_bytes >= (long long)need, kInvalidArgument, "workspace too small");
         p->ws = d_workspace;
       } else {
-        FC_HIP_CHECK(hipMalloc(&p->ws, wl.total()));
+        FC_HIP_CHECK(hipMalloc(&p->ws, need));
         p->own = true;
       }
-      char* base = static_cast<char*>(p->ws);
-      p->S = reinterpret_cast<float2*>(base);
-      p->Kx = reinterpret_cast<float2*>(base + wl.spectrum);
-      p->Kxy = reinterpret_cast<float2*>(base + wl.spectrum + wl.krows);
-      // twiddle tables now, so later executes allocate nothing
-      int lens[5];
-      for (int i = 0, n = table_lengths(s, lens); i < n; ++i) (void)fft_desc(lens[i]);
+      if (!lng) {
+        const WorkspaceLayout wl = workspace_layout(s);
+        char* base = static_cast<char*>(p->ws);
+        p->S = reinterpret_cast<float2*>(base);
+        p->Kx = reinterpret_cast<float2*>(base + wl.spectrum);
+        p->Kxy = reinterpret_cast<float2*>(base + wl.spectrum + wl.krows);
+        // twiddle tables now, so later executes allocate nothing (the long
+        // path creates its own with the kernel's spectrum)
+        int lens[5];
+        for (int i = 0, n = table_lengths(s, lens); i < n; ++i) (void)fft_desc(lens[i]);
+      }
       plan_kernel(*p, d_kernel, static_cast<hipStream_t>(stream));
     } catch (...) {
       if (p->own && p->ws) (void)hipFree(p->ws);
@@ -113,6 +128,10 @@ int fcamd_conv_plan_execute(void* plan, float* d_out, const float* d_image, void
     FC_REQUIRE(d_out && d_image, kInvalidArgument, "null device pointer");
     ConvPlan& p = *static_cast<ConvPlan*>(plan);
     hipStream_t st = static_cast<hipStream_t>(stream);
+    if (p.lng) {
+      long_apply_spectrum(p.s, d_out, d_image, p.op, p.ws, st);
+      return;
+    }
     if (plane_path(p.s, -1))
       plane_forward(p.s, d_image, p.S, nullptr, nullptr, st);
     else

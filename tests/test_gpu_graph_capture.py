@@ -25,7 +25,9 @@ def _replays(torch, g, src, data, outs_fn):
 
 
 @pytest.mark.parametrize("shape,kshape", [((64, 96, 128), (9, 7, 5)), ((40, 50, 60), (5, 3, 7)),
-                                          ((32, 128, 128), (7, 7, 7)), ((24, 256, 256), (9, 9, 9))])
+                                          ((32, 128, 128), (7, 7, 7)), ((24, 256, 256), (9, 9, 9)),
+                                          # long-axis path: chirp-z x, four-step x
+                                          ((8, 64, 4099), (3, 5, 7)), ((2, 3, 20000), (1, 3, 9))])
 def test_plan_and_device_call_in_cuda_graph(fc, oracle, shape, kshape):
     import torch
     from fourierconvolutioncudalib_amd import synth

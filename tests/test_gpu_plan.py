@@ -1,7 +1,9 @@
 """Convolution plans (kernel rows/planes transformed once): equal to the
 plain device call for every image, after a kernel change, for the
 correlation, on compile-time and runtime lengths, with a caller workspace,
-and in place."""
+and in place; on the long-axis path (axes past one tile, large prime
+factors) the plan keeps the kernel's whole spectrum and still equals the
+plain call."""
 import numpy as np
 import pytest
 
@@ -13,7 +15,12 @@ pytestmark = pytest.mark.gpu
                                           # wide planes: both directions split; forward
                                           # passes with the split inverse
                                           ((32, 256, 256), (9, 9, 9)),
-                                          ((128, 256, 256), (21, 21, 21))])
+                                          ((128, 256, 256), (21, 21, 21)),
+                                          # long-axis path: four-step x, one-tile
+                                          # chirp-z x, four-step chirp-z y
+                                          ((2, 3, 20000), (1, 3, 9)),
+                                          ((8, 64, 4099), (3, 5, 7)),
+                                          ((2, 10247, 3), (1, 9, 3))])
 def test_plan_equals_plain_calls(fc, oracle, shape, kshape):
     import torch
     from fourierconvolutioncudalib_amd import synth
@@ -55,4 +62,26 @@ def test_plan_kernel_change_correlate_workspace_inplace(fc):
     assert fc.lib().fcamd_last_error() == 1
     with pytest.raises(ValueError):
         plan.run(torch.zeros((40, 64, 71), device="cuda"))
+    plan.close()
+
+
+def test_long_axis_plan_kernel_change_correlate_inplace(fc, oracle):
+    """A long-axis plan (x = 4099, chirp-z) after set_kernel, as a
+    correlation, run in place on a caller workspace."""
+    import torch
+    from fourierconvolutioncudalib_amd import synth
+    shape, kshape = (6, 20, 4099), (3, 5, 7)
+    k1 = torch.from_numpy(synth.random_psf(kshape, 3)).cuda()
+    k2 = torch.from_numpy(synth.random_psf(kshape, 4)).cuda()
+    ws = torch.empty(fc.workspace_bytes(shape, kshape), dtype=torch.uint8, device="cuda")
+    plan = fc.ConvolutionPlan(shape, k1, op=fc.CORRELATE, workspace=ws)
+    a_np = synth.uniform(shape, 6)
+    a = torch.from_numpy(a_np).cuda()
+    assert torch.equal(plan.run(a), fc.convolve3d(a, k1, op=fc.CORRELATE))
+    plan.set_kernel(k2)
+    b = a.clone()
+    plan.run(b, out=b)
+    assert torch.equal(b, fc.convolve3d(a, k2, op=fc.CORRELATE))
+    ref = oracle.numpy_fft_conv(a_np, k2.cpu().numpy(), conj_kernel=True)
+    assert oracle.rel_l2(b.cpu().numpy(), ref) < 1e-5
     plan.close()
