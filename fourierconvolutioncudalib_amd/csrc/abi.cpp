@@ -506,7 +506,10 @@ int fcamd_convolve3d_boundary_host(float* im, const int* imDim, const float* ker
 
 long long fcamd_grad_kernel_workspace_bytes(const int* imDim, const int* kernelDim) {
   long long r = -1;
-  (void)guarded([&] { r = (long long)grad_workspace_layout(shape_from(imDim, kernelDim)).total(); });
+  (void)guarded([&] {
+    const ConvShape s = shape_from_any(imDim, kernelDim);
+    r = (long long)(needs_long_path(s) ? long_workspace_bytes(s) : grad_workspace_layout(s).total());
+  });
   return r;
 }
 
@@ -515,19 +518,26 @@ int fcamd_conv_grad_kernel_device(float* d_dkernel, const float* d_grad_out, con
                                   long long workspace_bytes, void* stream) {
   return guarded([&] {
     FC_REQUIRE(d_dkernel && d_grad_out && d_image, kInvalidArgument, "null device pointer");
-    ConvShape s = shape_from(imDim, kernelDim);
+    ConvShape s = shape_from_any(imDim, kernelDim);
     hipStream_t st = static_cast<hipStream_t>(stream);
-    const size_t need = grad_workspace_layout(s).total();
+    const bool lng = needs_long_path(s);
+    const size_t need = lng ? long_workspace_bytes(s) : grad_workspace_layout(s).total();
+    auto run = [&](void* ws) {
+      if (lng)
+        enqueue_grad_kernel_long(s, d_dkernel, d_grad_out, d_image, ws, st);
+      else
+        enqueue_grad_kernel(s, d_dkernel, d_grad_out, d_image, ws, st);
+    };
     if (d_workspace) {
       FC_REQUIRE(workspace_bytes >= (long long)need, kInvalidArgument, "workspace too small");
-      enqueue_grad_kernel(s, d_dkernel, d_grad_out, d_image, d_workspace, st);
+      run(d_workspace);
     } else {
       int dev = 0;
       FC_HIP_CHECK(hipGetDevice(&dev));
       DevicePool& pool = device_pool(dev);
       std::lock_guard<std::mutex> lk(pool.mu);
       void* ws = pool_reserve(&pool.ws, &pool.ws_bytes, need);
-      enqueue_grad_kernel(s, d_dkernel, d_grad_out, d_image, ws, st);
+      run(ws);
       FC_HIP_CHECK(hipStreamSynchronize(st));
     }
   });

@@ -172,6 +172,11 @@ void long_kernel_spectrum(const ConvShape& s, const float* d_kernel, void* ws,
                           hipStream_t stream);
 void long_apply_spectrum(const ConvShape& s, float* d_out, const float* d_image, Op op, void* ws,
                          hipStream_t stream);
+// The PSF gradient (enqueue_grad_kernel) on the long-axis path, in a
+// long_workspace_bytes(s) workspace: the cross-correlation of g and the
+// image through the unfused passes, then the gather of the support.
+void enqueue_grad_kernel_long(const ConvShape& s, float* d_dkernel, const float* d_grad_out,
+                              const float* d_image, void* ws, hipStream_t stream);
 // exp(-2 pi i j / L), j < L, on the current device (created once; a first
 // use while `stream` captures is refused like fft_desc's)
 const float2* long_twiddle_table(int L, hipStream_t stream);

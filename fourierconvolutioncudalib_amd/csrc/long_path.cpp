@@ -354,4 +354,21 @@ void enqueue_convolution_long(const ConvShape& s, float* d_out, const float* d_i
   long_apply_spectrum(s, d_out, d_image, op, ws, st);
 }
 
+void enqueue_grad_kernel_long(const ConvShape& s, float* d_dkernel, const float* d_grad_out,
+                              const float* d_image, void* ws, hipStream_t st) {
+  validate_long_shape(s);
+  driver_init();
+  prepare_tables(s, st);
+  const LongRegions r = long_regions(s, ws);
+  // u = IFFT(FFT(g) conj(FFT(im))) / N, the cross-correlation, into the
+  // kernel-volume region; dk[j] = u[(j - c) mod n] on the support
+  forward(s, d_grad_out, r.S, r.Z, r.W, st);
+  forward(s, d_image, r.K, r.Z, r.W, st);
+  const long n = (long)padded_bins(s.nx) * s.ny * s.nz;
+  launch_modulate(r.S, r.K, n, (float)(1.0 / (double)s.voxels()), 1, st);
+  inverse(s, r.S, r.kv, r.Z, r.W, st);
+  launch_gather_kernel_support(s, r.kv, d_dkernel, st);
+  FC_HIP_CHECK(hipGetLastError());
+}
+
 }  // namespace fcamd

@@ -54,7 +54,7 @@ def _fft64(a):
 
 def _corr_grad_kernel64(r, img, kshape):
     """FP64 dk[j] = sum_i r[i] img[i - (j - c)] via FFT cross-correlation."""
-    u = np.fft.irfftn(_fft64(r) * np.conj(_fft64(img)), s=r.shape)
+    u = np.fft.irfftn(_fft64(r) * np.conj(_fft64(img)), s=r.shape, axes=(0, 1, 2))
     idx = [(np.arange(k) - k // 2) % n for k, n in zip(kshape, r.shape)]
     return u[np.ix_(*idx)]
 
@@ -148,3 +148,30 @@ def test_autograd_gradcheck_small(fc):
                     fm = f(img.detach(), b.view_as(k))
             fd = float((fp - fm) / (2 * eps))
             assert abs(grad.view(-1)[idx].item() - fd) <= 2e-2 * max(1.0, abs(fd))
+
+
+@pytest.mark.parametrize("shape,kshape", [((6, 20, 4099), (3, 5, 7)),    # chirp-z x (one tile)
+                                          ((2, 3, 20000), (1, 3, 9)),    # four-step x
+                                          ((4099, 4, 6), (5, 3, 3))],    # chirp-z z
+                         ids=lambda v: "x".join(map(str, v)))
+def test_long_axis_gradients(fc, oracle, shape, kshape):
+    """Both adjoints on shapes of the long-axis path: the PSF gradient by the
+    unfused cross-correlation and the support gather, the image gradient by
+    the correlation, through torch autograd, against FP64."""
+    import torch
+    from fourierconvolutioncudalib_amd import synth
+    from fourierconvolutioncudalib_amd.autograd import fft_conv3d, grad_kernel
+    print("long-axis gradients", shape, kshape, flush=True)
+    rng = np.random.default_rng(sum(shape))
+    img = rng.random(shape, dtype=np.float32)
+    g = rng.standard_normal(shape).astype(np.float32)
+    psf = synth.random_psf(kshape, 7)
+    dk = grad_kernel(torch.from_numpy(g).cuda(), torch.from_numpy(img).cuda(), kshape)
+    ref = oracle.fft_conv_grad_kernel(g, img, kshape)
+    assert oracle.rel_l2(dk.cpu().numpy(), ref) < 1e-5
+    ti = torch.from_numpy(img).cuda().requires_grad_(True)
+    tk = torch.from_numpy(psf).cuda().requires_grad_(True)
+    (fft_conv3d(ti, tk) * torch.from_numpy(g).cuda()).sum().backward()
+    assert oracle.rel_l2(ti.grad.cpu().numpy(),
+                         oracle.numpy_fft_conv(g, psf, conj_kernel=True)) < 1e-5
+    assert oracle.rel_l2(tk.grad.cpu().numpy(), ref) < 1e-5
