@@ -27,13 +27,15 @@ std::atomic<bool> g_disabled{false};
 std::once_flag g_registered;
 }  // namespace
 
-// one per instantiation group of static_inst.hip (FC_GROUP 0..5)
+// one per instantiation group of static_inst.hip (FC_GROUP 0..7)
 void static_register_group_0();
 void static_register_group_1();
 void static_register_group_2();
 void static_register_group_3();
 void static_register_group_4();
 void static_register_group_5();
+void static_register_group_6();
+void static_register_group_7();
 
 namespace {
 // Fills the registry on first use.  Calling the groups by name (instead of
@@ -47,6 +49,8 @@ void ensure_registered() {
     static_register_group_3();
     static_register_group_4();
     static_register_group_5();
+    static_register_group_6();
+    static_register_group_7();
   });
 }
 }  // namespace
