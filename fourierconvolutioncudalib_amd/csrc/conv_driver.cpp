@@ -732,12 +732,17 @@ int fast_length(int n) {
       while (r % f == 0) r /= f;
     if (r == 1) break;
   }
-  // a compile-time length unless it is more than 25% longer
+  // a 7-smooth compile-time length unless it is more than 25% longer
+  auto smooth7 = [](int m) {
+    for (int f : {2, 3, 5, 7})
+      while (m % f == 0) m /= f;
+    return m == 1;
+  };
   int best = 0;
   int lens[64];
   const int cnt = fcamd_static_lengths(lens, 64);
   for (int i = 0; i < cnt && i < 64; ++i)
-    if (lens[i] >= n && (best == 0 || lens[i] < best)) best = lens[i];
+    if (lens[i] >= n && smooth7(lens[i]) && (best == 0 || lens[i] < best)) best = lens[i];
   return best && 4L * best <= 5L * smooth ? best : smooth;
 }
 

@@ -294,4 +294,47 @@ struct Dft<16, DIR> {
   }
 };
 
+// Odd prime radix R as a direct DFT over the pair sums and differences
+// a_n = x_n + x_(R-n), b_n = x_n - x_(R-n), n = 1..(R-1)/2:
+//   y_k = x_0 + sum_n a_n cos(2 pi n k / R) + DIR i sum_n b_n sin(2 pi n k / R)
+// and y_(R-k) the same with the sine sum subtracted (the radix-7 form above
+// for any odd prime).
+template <int R, int DIR>
+struct PrimeDft {
+  __device__ __forceinline__ static void run(float2* v) {
+    constexpr int H = (R - 1) / 2;
+    constexpr cx::Roots<R> W{};
+    float2 a[H + 1], b[H + 1];
+    const float2 x0 = v[0];
+    float2 sum = x0;
+#pragma unroll
+    for (int n = 1; n <= H; ++n) {
+      a[n] = cadd(v[n], v[R - n]);
+      b[n] = csub(v[n], v[R - n]);
+      sum = cadd(sum, a[n]);
+    }
+#pragma unroll
+    for (int k = 1; k <= H; ++k) {
+      float2 re = x0, im = make_float2(0.f, 0.f);
+#pragma unroll
+      for (int n = 1; n <= H; ++n) {
+        const int m = (n * k) % R;
+        re.x += W.c[m] * a[n].x;
+        re.y += W.c[m] * a[n].y;
+        im.x += W.s[m] * b[n].x;
+        im.y += W.s[m] * b[n].y;
+      }
+      const float2 i = rot90<DIR>(im);
+      v[k] = cadd(re, i);
+      v[R - k] = csub(re, i);
+    }
+    v[0] = sum;
+  }
+};
+
+template <int DIR>
+struct Dft<11, DIR> : PrimeDft<11, DIR> {};
+template <int DIR>
+struct Dft<13, DIR> : PrimeDft<13, DIR> {};
+
 }  // namespace fcamd

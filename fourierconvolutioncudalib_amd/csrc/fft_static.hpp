@@ -30,9 +30,9 @@ struct StaticPlanData {
 #define FC_POW2_MAXBITS 4
 #endif
 
-// Classic plan for a 7-smooth length: the power-of-two part in the fewest
+// Classic plan for a 13-smooth length: the power-of-two part in the fewest
 // stages of radix <= 16 with the bits spread evenly (256 = 16*16,
-// 512 = 8*8*8, 1024 = 16*8*8, 128 = 16*8), then 3s, 5s and 7s; one thread
+// 512 = 8*8*8, 1024 = 16*8*8, 128 = 16*8), then 3s, 5s, 7s, 11s and 13s; one thread
 // per largest-radix butterfly.
 constexpr StaticPlanData classic_plan(int L) {
   StaticPlanData p{};
@@ -47,8 +47,10 @@ constexpr StaticPlanData classic_plan(int L) {
   while (n % 3 == 0) { p.r[p.n++] = 3; n /= 3; }
   while (n % 5 == 0) { p.r[p.n++] = 5; n /= 5; }
   while (n % 7 == 0) { p.r[p.n++] = 7; n /= 7; }
+  while (n % 11 == 0) { p.r[p.n++] = 11; n /= 11; }
+  while (n % 13 == 0) { p.r[p.n++] = 13; n /= 13; }
   if (n != 1) {
-    p.n = -1;  // not 7-smooth: no static plan
+    p.n = -1;  // not 13-smooth: no static plan
     return p;
   }
   int rmax = 1;
@@ -66,7 +68,7 @@ constexpr StaticPlanData classic_plan(int L) {
 // twiddle and LDS-exchange work.  At most 32 values per thread are held in
 // registers.
 namespace plan_detail {
-constexpr int kRadix[13] = {16, 15, 14, 12, 10, 9, 8, 7, 6, 5, 4, 3, 2};
+constexpr int kRadix[15] = {16, 15, 14, 13, 12, 11, 10, 9, 8, 7, 6, 5, 4, 3, 2};
 constexpr int butterfly_flops(int r) {
   switch (r) {
     case 2: return 4;
@@ -78,6 +80,8 @@ constexpr int butterfly_flops(int r) {
     case 8: return 52;
     case 9: return 120;
     case 10: return 124;
+    case 11: return 240;  // direct pair form: 2 * 5 * 5 complex multiply-adds
+    case 13: return 340;
     case 12: return 148;
     case 14: return 208;
     case 15: return 248;
@@ -114,7 +118,7 @@ constexpr void search(int rem, int first, StaticPlanData& cur, int S, int L, Bes
     if (rem == 1) consider(cur, L, best);
     return;
   }
-  for (int i = first; i < 13; ++i) {
+  for (int i = first; i < 15; ++i) {
     const int R = kRadix[i];
     if (rem % R != 0) continue;
     cur.r[cur.n++] = R;

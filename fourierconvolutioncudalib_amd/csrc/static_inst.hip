@@ -24,7 +24,7 @@
 #elif FC_GROUP == 5
 #define FC_LENGTHS(X) X(32) X(48) X(288) X(320) X(400) X(448) X(640) X(768)
 #elif FC_GROUP == 6
-#define FC_LENGTHS(X) X(100) X(150) X(200) X(250) X(300) X(360) X(480)
+#define FC_LENGTHS(X) X(100) X(150) X(200) X(250) X(286) X(300) X(360) X(480)
 #elif FC_GROUP == 7
 #define FC_LENGTHS(X) X(560) X(700) X(720) X(800) X(900) X(960) X(1000)
 #else
