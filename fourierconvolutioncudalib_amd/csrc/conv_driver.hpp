@@ -152,7 +152,8 @@ struct PaddedProblem {
   VPad vp;
 };
 PaddedProblem padded_problem(const ConvShape& s, int boundary);
-// smallest fast transform length >= n (a compile-time length when one is)
+// smallest fast transform length >= n (a 7-smooth compile-time length when
+// one is within 25 %, else the smallest 7-smooth length)
 int fast_length(int n);
 void enqueue_convolution_padded(const PaddedProblem& p, float* d_out, const float* d_image,
                                 const float* d_kernel, Op op, void* ws, hipStream_t stream);
