@@ -27,6 +27,8 @@
 #define FC_LENGTHS(X) X(100) X(150) X(200) X(250) X(286) X(300) X(360) X(480)
 #elif FC_GROUP == 7
 #define FC_LENGTHS(X) X(560) X(700) X(720) X(800) X(900) X(960) X(1000)
+#elif FC_GROUP == 8
+#define FC_LENGTHS(X) X(180) X(216) X(270) X(336) X(432) X(540) X(576) X(672) X(864) X(896)
 #else
 #define FC_LENGTHS(X)
 #endif

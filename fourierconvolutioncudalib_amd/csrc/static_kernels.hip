@@ -27,7 +27,7 @@ std::atomic<bool> g_disabled{false};
 std::once_flag g_registered;
 }  // namespace
 
-// one per instantiation group of static_inst.hip (FC_GROUP 0..7)
+// one per instantiation group of static_inst.hip (FC_GROUP 0..8)
 void static_register_group_0();
 void static_register_group_1();
 void static_register_group_2();
@@ -36,6 +36,7 @@ void static_register_group_4();
 void static_register_group_5();
 void static_register_group_6();
 void static_register_group_7();
+void static_register_group_8();
 
 namespace {
 // Fills the registry on first use.  Calling the groups by name (instead of
@@ -51,6 +52,7 @@ void ensure_registered() {
     static_register_group_5();
     static_register_group_6();
     static_register_group_7();
+    static_register_group_8();
   });
 }
 }  // namespace
