@@ -122,7 +122,31 @@ size_t default_wave_budget() {
   return kDefaultWaveBytes;
 }
 
-size_t block_workspace(const ConvShape& s) { return align256(workspace_layout(s).total()); }
+// Blocks the drop-in call would send to the long-axis path (axes past one
+// tile, large prime factors) run there in the lane schedule; the length
+// groups take one-tile blocks only.
+size_t block_workspace(const ConvShape& s) {
+  return align256(needs_long_path(s) ? long_workspace_bytes(s) : workspace_layout(s).total());
+}
+
+void validate_block(const ConvShape& s, int nlanes) {
+  if (!needs_long_path(s)) {
+    validate_shape(s);
+    return;
+  }
+  validate_long_shape(s);
+  FC_REQUIRE(nlanes > 0, kUnsupportedSize,
+             "a block needs the long-axis path, which the length-group schedule "
+             "(FCAMD_BATCH_LANES=0) does not run");
+}
+
+void enqueue_block(const ConvShape& s, float* out, const float* img, const float* ker, Op op,
+                   void* ws, hipStream_t st) {
+  if (needs_long_path(s))
+    enqueue_convolution_long(s, out, img, ker, op, ws, st);
+  else
+    enqueue_convolution(s, out, img, ker, op, ws, st);
+}
 
 // Lane count: FCAMD_BATCH_LANES when set (0 = length groups), else the
 // default above.
@@ -229,7 +253,7 @@ BatchPlan::BatchPlan(int n, const int* imDims, const int* kernelDims, float* con
   for (int b = 0; b < n; ++b) {
     Block bl{};
     bl.s = block_shape(imDims, kernelDims, b);
-    validate_shape(bl.s);
+    validate_block(bl.s, nlanes);
     FC_REQUIRE(outs[b] && imgs[b] && kers[b], kInvalidArgument, "null block pointer");
     bl.out = outs[b];
     bl.img = imgs[b];
@@ -262,6 +286,10 @@ BatchPlan::BatchPlan(int n, const int* imDims, const int* kernelDims, float* con
   // create every twiddle table now: nothing may allocate or copy while a
   // graph is being captured
   for (const Block& bl : blocks_) {
+    if (needs_long_path(bl.s)) {
+      long_prepare_tables(bl.s, nullptr);
+      continue;
+    }
     int lens[5];
     for (int i = 0, n = table_lengths(bl.s, lens); i < n; ++i) (void)fft_desc(lens[i]);
   }
@@ -551,8 +579,8 @@ void BatchPlan::enqueue_lanes(hipStream_t st) {
     for (size_t i = 0; i < nl; ++i) {
       if (j >= lanes_[i].size()) continue;
       const Block& bl = blocks_[lanes_[i][j]];
-      enqueue_convolution(bl.s, bl.out, bl.img, bl.ker, op, static_cast<char*>(ws_) + lane_off_[i],
-                          i == 0 ? st : side_[i - 1]);
+      enqueue_block(bl.s, bl.out, bl.img, bl.ker, op, static_cast<char*>(ws_) + lane_off_[i],
+                    i == 0 ? st : side_[i - 1]);
     }
   }
   for (size_t i = 1; i < nl; ++i) {
@@ -601,8 +629,8 @@ void BatchPlan::graph_launch(hipStream_t st) {
           try {
             for (int b : lanes_[i]) {
               const Block& bl = blocks_[b];
-              enqueue_convolution(bl.s, bl.out, bl.img, bl.ker, op,
-                                  static_cast<char*>(ws_) + lane_off_[i], cs);
+              enqueue_block(bl.s, bl.out, bl.img, bl.ker, op,
+                            static_cast<char*>(ws_) + lane_off_[i], cs);
             }
           } catch (...) {
             hipGraph_t g = nullptr;
@@ -701,12 +729,13 @@ long long fcamd_batch_workspace_bytes(int nblocks, const int* imDims, const int*
     std::vector<size_t> sizes;
     std::vector<ConvShape> shapes;
     for (int b = 0; b < nblocks; ++b) {
-      const ConvShape s = block_shape(imDims, kernelDims, b);
-      validate_shape(s);
-      sizes.push_back(block_workspace(s));
-      shapes.push_back(s);
+      shapes.push_back(block_shape(imDims, kernelDims, b));
     }
     const int nlanes = default_lanes();
+    for (const ConvShape& s : shapes) {
+      validate_block(s, nlanes);
+      sizes.push_back(block_workspace(s));
+    }
     size_t need = 0;
     if (nlanes > 0) {
       std::vector<size_t> slot;

@@ -162,6 +162,9 @@ void enqueue_convolution_padded(const PaddedProblem& p, float* d_out, const floa
 // z > 6826) take the unfused long-axis path (long_path.cpp): four-step FFTs
 // through HBM for such axes, the kernel placed on the image grid.
 bool needs_long_path(const ConvShape& s);
+// every table the long-axis path reads for shape s, created now (a first use
+// while `stream` captures is refused)
+void long_prepare_tables(const ConvShape& s, hipStream_t stream);
 void validate_long_shape(const ConvShape& s);
 size_t long_workspace_bytes(const ConvShape& s);
 void enqueue_convolution_long(const ConvShape& s, float* d_out, const float* d_image,

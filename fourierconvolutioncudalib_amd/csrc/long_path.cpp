@@ -284,6 +284,11 @@ void prepare_tables(const ConvShape& s, hipStream_t st) {
 
 }  // namespace
 
+void long_prepare_tables(const ConvShape& s, hipStream_t st) {
+  validate_long_shape(s);
+  prepare_tables(s, st);
+}
+
 bool needs_long_path(const ConvShape& s) {
   return s.nx > kTileLen - 1 || s.ny > kTileLen || s.nz > 6826 || s.ny > 65535 ||
          s.nz > 65535 || chirp_length(s.nx) || chirp_length(s.ny) || chirp_length(s.nz);

@@ -185,3 +185,43 @@ def test_batched_in_place_mixed_runs(fc, oracle, schedule):
         for g, w in zip(ti, want):
             assert oracle.rel_l2(g.cpu().numpy(), w.cpu().numpy()) < 1e-6, (rep, graph)
     plan.close()
+
+
+LONG_SHAPES = [((2, 3, 20000), (1, 3, 9)),     # x past one tile: four-step
+               ((8, 64, 4099), (3, 5, 7)),     # prime x: chirp-z
+               ((64, 80, 96), (9, 11, 13)),
+               ((36, 50, 13), (5, 7, 3))]
+
+
+def test_batched_lanes_with_long_axis_blocks(fc, oracle, monkeypatch):
+    """Blocks the drop-in call sends to the long-axis path run there in the
+    lane schedule, beside one-tile blocks: eager and over graph replays,
+    equal to the plain calls and to FP64.  The length-group schedule refuses
+    them before any device work."""
+    import ctypes
+    import torch
+    from fourierconvolutioncudalib_amd.batched import BatchedConvolution
+    monkeypatch.setenv("FCAMD_BATCH_LANES", "4")
+    imgs, ks = _blocks(900, LONG_SHAPES)
+    ti = [torch.from_numpy(a).cuda() for a in imgs]
+    tk = [torch.from_numpy(k).cuda() for k in ks]
+    plan = BatchedConvolution(ti, tk)
+    plain = [fc.convolve3d(a, k).cpu().numpy() for a, k in zip(ti, tk)]
+    for e, p, a, k in zip([o.cpu().numpy() for o in plan.run(graph=False)], plain, imgs, ks):
+        assert oracle.rel_l2(e, p) < 1e-6
+        assert oracle.rel_l2(e, oracle.numpy_fft_conv(a, k)) < TOL
+    for rep in range(3):
+        for o in plan.outs:
+            o.fill_(float("nan"))
+        torch.cuda.synchronize()
+        got = [o.clone() for o in plan.run(graph=True)]
+        torch.cuda.synchronize()
+        for g, p in zip(got, plain):
+            assert oracle.rel_l2(g.cpu().numpy(), p) < 1e-6, rep
+    plan.close()
+    n = len(imgs)
+    dims = (ctypes.c_int * (3 * n))(*[d for a in imgs for d in a.shape[::-1]])
+    kdims = (ctypes.c_int * (3 * n))(*[d for k in ks for d in k.shape[::-1]])
+    monkeypatch.setenv("FCAMD_BATCH_LANES", "0")
+    assert fc.lib().fcamd_batch_workspace_bytes(n, dims, kdims) < 0
+    assert fc.lib().fcamd_last_error() == 2
