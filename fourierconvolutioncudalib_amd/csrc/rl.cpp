@@ -57,7 +57,12 @@ RLProblem make_problem(int nviews, const int* imDim, const int* kernelDims) {
     s.cx = s.kx / 2;
     s.cy = s.ky / 2;
     s.cz = s.kz / 2;
-    validate_shape(s);
+    // shapes the drop-in call sends to the long-axis path take it in the
+    // unfused form (the fused kernels exist for compile-time lengths only)
+    if (needs_long_path(s))
+      validate_long_shape(s);
+    else
+      validate_shape(s);
     p.views.push_back(s);
   }
   return p;
@@ -91,7 +96,8 @@ RLLayout rl_layout(const RLProblem& p) {
     const ConvShape& s = p.views[v];
     l.krows[v] = a256(sizeof(float2) * nxp * s.ky * s.kz);
     l.kplanes[v] = a256(sizeof(float2) * nxp * s.ny * s.kz);
-    l.conv = std::max(l.conv, workspace_layout(s).total());
+    l.conv = std::max(l.conv, needs_long_path(s) ? long_workspace_bytes(s)
+                                                 : workspace_layout(s).total());
   }
   l.vol = a256(sizeof(float) * (size_t)p.img.voxels());
   return l;
@@ -121,11 +127,17 @@ void run_rl(const RLProblem& p, float* psi, const float* const* views, const flo
     float* blur = reinterpret_cast<float*>(take(l.vol));
     float* ratio = reinterpret_cast<float*>(take(l.vol));
     float* acc = reinterpret_cast<float*>(take(l.vol));
+    auto conv = [&](const ConvShape& s, float* out, const float* in, const float* k, Op op) {
+      if (needs_long_path(s))
+        enqueue_convolution_long(s, out, in, k, op, cws, st);
+      else
+        enqueue_convolution(s, out, in, k, op, cws, st);
+    };
     for (int it = 0; it < iterations; ++it) {
       for (int v = 0; v < p.V; ++v) {
-        enqueue_convolution(p.views[v], blur, psi, kernels[v], Op::kConvolve, cws, st);
+        conv(p.views[v], blur, psi, kernels[v], Op::kConvolve);
         launch_rl_ratio(ratio, views[v], blur, eps, n, st);
-        enqueue_convolution(p.views[v], blur, ratio, kernels[v], Op::kCorrelate, cws, st);
+        conv(p.views[v], blur, ratio, kernels[v], Op::kCorrelate);
         launch_rl_accumulate(acc, blur, v == 0 ? 1 : 0, n, st);
       }
       launch_rl_update(psi, acc, 1.f / (float)p.V, n, st);

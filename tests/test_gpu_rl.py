@@ -39,6 +39,21 @@ def test_rl_vs_fp64_oracle(fc, oracle, shape, kshape):
         assert oracle.rel_l2(_run(views, psfs, 10, True), _run(views, psfs, 10, False)) < 1e-5
 
 
+def test_rl_long_axis_shape(fc, oracle):
+    """A prime x past the fused Bluestein (4099, long-axis chirp-z) in the
+    unfused form, against the FP64 oracle."""
+    import torch
+    from fourierconvolutioncudalib_amd import rl, synth
+    shape, kshape = (4, 6, 4099), (3, 5, 9)
+    _, views, psfs = synth.multiview(shape, kshape, 2, 78, sigma_xyz=(1.5, 1.0, 1.0),
+                                     blobs_count=40)
+    ref = oracle.rl_multiview(np.stack(views), psfs, 10, eps=1e-6)
+    tv = [torch.from_numpy(v).cuda() for v in views]
+    tk = [torch.from_numpy(k).cuda() for k in psfs]
+    assert not rl.fused_available(tv, tk)
+    assert oracle.rel_l2(_run(views, psfs, 10, False), ref) < 1e-4
+
+
 def test_rl_multiview_384x384x192(fc, oracle):
     """4 views of 384x384x192 with 25x25x51 PSFs: fused vs per-view GPU forms
     over 10 iterations, flux conservation, and the first two iterations
