@@ -40,11 +40,13 @@ def test_rl_vs_fp64_oracle(fc, oracle, shape, kshape):
 
 
 def test_rl_long_axis_shape(fc, oracle):
-    """A prime x past the fused Bluestein (4099, long-axis chirp-z) in the
-    unfused form, against the FP64 oracle."""
+    """An x axis past one tile (10240, the long-axis path's four-step) in the
+    unfused form, against the FP64 oracle (whose own transform of a large
+    prime length would be quadratic; the chirp-z axes are checked by the
+    convolution tests)."""
     import torch
     from fourierconvolutioncudalib_amd import rl, synth
-    shape, kshape = (4, 6, 4099), (3, 5, 9)
+    shape, kshape = (3, 6, 10240), (3, 5, 9)
     _, views, psfs = synth.multiview(shape, kshape, 2, 78, sigma_xyz=(1.5, 1.0, 1.0),
                                      blobs_count=40)
     ref = oracle.rl_multiview(np.stack(views), psfs, 10, eps=1e-6)
