@@ -206,3 +206,15 @@ def test_device_queries_without_device(fc):
                                           k.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
                                           fc._lib.ints([3, 3, 3]), 0, 0) == 5
     assert np.all(im == 1)
+
+
+def test_boundary_padding_lengths(fc):
+    """Boundary modes pad each axis with a kernel to a fast length >= n + k - 1:
+    the smallest 7-smooth compile-time length within 25 % of the smallest
+    7-smooth length (host arithmetic only).  526 -> 540 and 270 -> 270 at
+    512^2 x 256 with 15^3; 272 -> 288, skipping the registered 286 = 2*11*13
+    (its radix-11/13 plan is slower than a 7-smooth one)."""
+    assert fc.padded_extents((256, 512, 512), (15, 15, 15), "zero") == (270, 540, 540)
+    assert fc.padded_extents((1, 1, 260), (1, 1, 13), "zero") == (1, 1, 288)
+    assert fc.padded_extents((1, 1, 274), (1, 1, 13), "reflect") == (1, 1, 288)
+    assert 286 in fc.static_lengths()
