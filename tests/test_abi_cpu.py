@@ -213,7 +213,7 @@ def test_boundary_padding_lengths(fc):
     the smallest 7-smooth compile-time length within 25 % of the smallest
     7-smooth length (host arithmetic only).  526 -> 540 and 270 -> 270 at
     512^2 x 256 with 15^3; 272 -> 288, skipping the registered 286 = 2*11*13
-    (its radix-11/13 plan is slower than a 7-smooth one)."""
+    (padding keeps to 7-smooth lengths)."""
     assert fc.padded_extents((256, 512, 512), (15, 15, 15), "zero") == (270, 540, 540)
     assert fc.padded_extents((1, 1, 260), (1, 1, 13), "zero") == (1, 1, 288)
     assert fc.padded_extents((1, 1, 274), (1, 1, 13), "reflect") == (1, 1, 288)
