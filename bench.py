@@ -347,16 +347,21 @@ def boundary_comparison(fc, torch, img_np, psf_np, checks, reps=10):
 # (tests/test_gpu_numerical_stability.cpp:40-96, :330-382): every extent
 # carries a large prime factor (79, 109, 11 and 13, 173)
 PRIME_SHAPES = [((218, 158, 158), (91, 31, 31)), ((346, 286, 286), (91, 31, 31))]
+# shapes of the long-axis path: a prime x whose one-tile plan would be a
+# direct 4099-point stage (chirp-z over M = 8232), and x past one tile
+# (four-step 125 x 160)
+LONG_AXIS_SHAPES = [((8, 64, 4099), (3, 5, 7)), ((4, 16, 20000), (3, 5, 9))]
+LONG_AXIS_PATHS = ["x: chirp-z over M = 8232 in one tile", "x: four-step 125 x 160"]
 
 
-def prime_shapes_timing(fc, torch, checks, reps=10):
-    """The reference's prime-padded test shapes, device-resident, seeded
-    uniform images with Gaussian PSFs: time per convolution and roofline
-    fraction (these lengths run the runtime-length kernels, prime factors as
-    direct DFT stages)."""
+def prime_shapes_timing(fc, torch, checks, reps=10, shapes=PRIME_SHAPES, label="prime shape"):
+    """The reference's prime-padded test shapes (or `shapes`), device-resident,
+    seeded uniform images with Gaussian PSFs: time per convolution and
+    roofline fraction.  158 = 2 * 79 and the z lengths run Bluestein on the
+    runtime-length kernels, 286 = 2 * 11 * 13 the compile-time kernels."""
     from fourierconvolutioncudalib_amd import synth
     out = []
-    for shape, kshape in PRIME_SHAPES:
+    for shape, kshape in shapes:
         img_np = synth.uniform(shape, 1500 + shape[0])
         psf_np = synth.gaussian_psf(kshape, (2.0, 2.0, 6.0))
         img = torch.from_numpy(img_np).cuda()
@@ -368,7 +373,7 @@ def prime_shapes_timing(fc, torch, checks, reps=10):
                                                                 stream=stream), reps)
         got = res.cpu().numpy()
         name = "x".join(map(str, shape[::-1])) + "_k" + "x".join(map(str, kshape[::-1]))
-        checks.add(f"prime shape {name}", lambda O, got=got, a=img_np, k=psf_np:
+        checks.add(f"{label} {name}", lambda O, got=got, a=img_np, k=psf_np:
                    (got, O.numpy_fft_conv(a, k)))
         nz, ny, nx = shape
         kz, ky, kx = kshape
@@ -377,7 +382,17 @@ def prime_shapes_timing(fc, torch, checks, reps=10):
                                                           fc.radix_plan(nz)],
                     "ms_per_conv": ms, "voxels_per_s": nx * ny * nz / (ms * 1e-3),
                     "conv_roofline_frac": cb / (ms * 1e-3) / HBM_PEAK})
-        log(f"prime shape {name}: {ms:.3f} ms")
+        log(f"{label} {name}: {ms:.3f} ms")
+    return out
+
+
+def long_axis_timing(fc, torch, checks, reps=10):
+    """The long-axis path (chirp-z and four-step), timed and checked like
+    the prime shapes."""
+    out = prime_shapes_timing(fc, torch, checks, reps, LONG_AXIS_SHAPES, "long-axis shape")
+    for r, path in zip(out, LONG_AXIS_PATHS):
+        del r["radix_plans_xyz"]  # the one-tile plans, which this path does not run
+        r["path"] = path
     return out
 
 
@@ -486,7 +501,7 @@ def main():
                     help="no tuning: apply the variant choices recorded in FILE by an earlier "
                          "run with FCAMD_TUNE_CACHE=FILE (so that a profiled run uses the "
                          "kernels an unprofiled one chose)")
-    ap.add_argument("--only", choices=["batched", "rl", "adjoint", "configs", "primes"],
+    ap.add_argument("--only", choices=["batched", "rl", "adjoint", "configs", "primes", "longaxis"],
                     help="run one side measurement alone and print its JSON (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -510,6 +525,7 @@ def main():
             "configs": lambda: [config_timing(fc, torch, synth.CFG1, 50, ck, jobs, None, True),
                                 config_timing(fc, torch, synth.CFG2, 20, ck, jobs, 16, False)],
             "primes": lambda: prime_shapes_timing(fc, torch, ck),
+            "longaxis": lambda: long_axis_timing(fc, torch, ck),
         }
         print(json.dumps({args.only: sides[args.only]()}))
         return
@@ -691,6 +707,7 @@ def main():
             config_timing(fc, torch, synth.CFG1, 50, checks, cpu_jobs, None, True),
             config_timing(fc, torch, synth.CFG2, 20, checks, cpu_jobs, 16, False)]
         result["reference_prime_shapes"] = prime_shapes_timing(fc, torch, checks)
+        result["long_axis_shapes"] = long_axis_timing(fc, torch, checks)
     del img, psf, out, ws
     torch.cuda.empty_cache()
 
