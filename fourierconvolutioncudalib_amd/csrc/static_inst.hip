@@ -29,6 +29,8 @@
 #define FC_LENGTHS(X) X(560) X(700) X(720) X(800) X(900) X(960) X(1000)
 #elif FC_GROUP == 8
 #define FC_LENGTHS(X) X(180) X(216) X(270) X(336) X(432) X(540) X(576) X(672) X(864) X(896)
+#elif FC_GROUP == 9
+#define FC_LENGTHS(X) X(1152) X(1200) X(1280) X(1536) X(2048)
 #else
 #define FC_LENGTHS(X)
 #endif
