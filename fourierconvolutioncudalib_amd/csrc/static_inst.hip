@@ -31,6 +31,8 @@
 #define FC_LENGTHS(X) X(180) X(216) X(270) X(336) X(432) X(540) X(576) X(672) X(864) X(896)
 #elif FC_GROUP == 9
 #define FC_LENGTHS(X) X(1152) X(1200) X(1280) X(1536) X(2048)
+#elif FC_GROUP == 10
+#define FC_LENGTHS(X) X(2304) X(2560) X(3072) X(4096)
 #else
 #define FC_LENGTHS(X)
 #endif

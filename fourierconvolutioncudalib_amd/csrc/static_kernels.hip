@@ -27,7 +27,7 @@ std::atomic<bool> g_disabled{false};
 std::once_flag g_registered;
 }  // namespace
 
-// one per instantiation group of static_inst.hip (FC_GROUP 0..9)
+// one per instantiation group of static_inst.hip (FC_GROUP 0..10)
 void static_register_group_0();
 void static_register_group_1();
 void static_register_group_2();
@@ -38,6 +38,7 @@ void static_register_group_6();
 void static_register_group_7();
 void static_register_group_8();
 void static_register_group_9();
+void static_register_group_10();
 
 namespace {
 // Fills the registry on first use.  Calling the groups by name (instead of
@@ -55,6 +56,7 @@ void ensure_registered() {
     static_register_group_7();
     static_register_group_8();
     static_register_group_9();
+    static_register_group_10();
   });
 }
 }  // namespace
