@@ -739,9 +739,9 @@ int fast_length(int n) {
     return m == 1;
   };
   int best = 0;
-  int lens[64];
-  const int cnt = fcamd_static_lengths(lens, 64);
-  for (int i = 0; i < cnt && i < 64; ++i)
+  int lens[128];
+  const int cnt = fcamd_static_lengths(lens, 128);
+  for (int i = 0; i < cnt && i < 128; ++i)
     if (lens[i] >= n && smooth7(lens[i]) && (best == 0 || lens[i] < best)) best = lens[i];
   return best && 4L * best <= 5L * smooth ? best : smooth;
 }

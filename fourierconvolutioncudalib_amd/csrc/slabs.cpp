@@ -118,9 +118,9 @@ SlabPlan slab_plan(const ConvShape& s, size_t budget, bool force) {
   const size_t bytes = plane_bytes(s) * (size_t)s.nz;
   if ((!force && bytes < kSlabMinBytes) || s.kz < 1) return best;
   const int min_slabs = force ? 2 : 4;
-  int lens[64];
-  const int cnt = fcamd_static_lengths(lens, 64);
-  for (int i = 0; i < cnt && i < 64; ++i) {
+  int lens[128];
+  const int cnt = fcamd_static_lengths(lens, 128);
+  for (int i = 0; i < cnt && i < 128; ++i) {
     const int m0 = lens[i] - (s.kz - 1);
     if (m0 < 8 || m0 < 2 * (s.kz - 1)) continue;
     const int n = (s.nz + m0 - 1) / m0;
